@@ -1,0 +1,186 @@
+"""Benchmark of the Lomb-Scargle hot path on one MI355X.
+
+Workload: the standard L-S batch of 1000 objects with log-normal observation
+counts (median 140, sigma 0.6, clipped to 20..1000; ~165k rows) over 10^6
+angular frequencies 1.005..150.796 rad/day, inputs resident in HBM (FP64
+arrays), periodogram written to HBM and the per-object best period fused on
+the device.  A step is one full sweep (statistics, records, sweep kernel,
+peak reduction).
+
+Reported (one JSON line on stdout):
+  value           freq*obs terms/s of the FP32-split form (FP64 phase
+                  reduction, FP32 sin/cos and sums, FP64 power)
+  fp64_plain      the same workload in the reference's FP64 arithmetic
+  objects_per_s   objects per second of the split form
+  accuracy        split vs FP64 over all 10^9 bins (bins with P >= 1e-3 Pmax)
+  roofline        the sweep kernel's VALU work per launch / its duration
+                  (HIP events on the launch stream) against the FP32 VALU peak
+  cpu_baseline    the reference's own CPU batch path (mode 4, OpenMP) timed on
+                  the host on a bounded object sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# VALU issue model (FP32 lane-cycles per freq*obs term, canonical algorithm:
+# 1 phase multiply + sin + cos + 5 FMA into XC, XS, CC, SS, CS), with the
+# transcendental cost measured on gfx950 (v_sin_f32 / v_cos_f32 issue at a
+# quarter of the FMA rate).
+TRANS_CYCLES = 4.0
+CYCLES_PER_TERM = 1.0 + 2.0 * TRANS_CYCLES + 5.0
+# MI355X FP32 vector peak without packed FP32 (disabled for this build; sin
+# and cos have no packed form): 256 CU x 4 SIMD x 16 lanes x 2.4 GHz lane-cycles/s
+PEAK_LANE_CYCLES = 256 * 4 * 16 * 2.4e9
+PEAK_TFLOPS = 2.0 * PEAK_LANE_CYCLES / 1e12  # 78.6 TFLOP/s (FMA = 2 flops)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(data, nf, sample_objects, threads):
+    """Reference CPU batch (MODE 4) on the first `sample_objects` objects."""
+    import oracle
+    counts = data["counts"][:sample_objects]
+    rows = int(counts.sum())
+    o, t, y = data["obj_id"][:rows], data["t"][:rows], data["y"][:rows]
+    kind = "reference" if oracle.ref_available(False, np.float64, True, threads) else "port"
+    t0 = time.perf_counter()
+    if kind == "reference":
+        oracle.ref_batch(o, t, y, None, 1.005, 150.796, nf, False, np.float64, threads=threads)
+    else:
+        oracle.batch(o, t, y, None, 1.005, 150.796, nf, False, np.float64, nthreads=threads)
+    dt = time.perf_counter() - t0
+    terms = float(rows) * nf
+    return {"value": terms / dt, "unit": "terms/s", "cores": threads, "kind": kind,
+            "sample": f"{sample_objects} of the 1000 objects ({rows} rows) x {nf} freqs, FP64, "
+                      f"reference MODE 4 (OpenMP dynamic over objects), {dt:.1f} s",
+            "objects_per_s": sample_objects / dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-objects", type=int, default=24)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nf", type=int, default=1000000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(objects shard across GPUs without any exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from gpu_lomb_scargle_amd import _native, device, synth
+
+    dev = torch.device("cuda:0")
+    L = _native.core()
+    data = synth.standard_batch_1000()
+    K = int(data["counts"].size)
+    rows = int(data["counts"].sum())
+    nf = args.nf
+    terms = float(rows) * nf
+    log(f"workload: {K} objects, {rows} rows, {nf} freqs, {terms:.3e} terms/step")
+    t, y, dy, off = device.numpy_to_device(data["obj_id"], data["t"], data["y"], None, np.float64, dev)
+
+    def make(form):
+        return device.SweepPlan(t, y, None, off, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
+                                num_freqs=nf, variant=_native.STANDARD, form=form, normalize=True,
+                                top_k=1)
+
+    def timed(plan, steps, warmup):
+        for _ in range(warmup):
+            plan.run()
+        torch.cuda.synchronize()
+        L.lsamd_set_timing(1)
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.run()
+            torch.cuda.synchronize()  # one step = one sweep; sync so the kernel time is read per step
+            kms.append(L.lsamd_last_sweep_kernel_ms())
+        t1 = time.perf_counter()
+        L.lsamd_set_timing(0)
+        return (t1 - t0) * 1e3 / steps, float(np.mean(kms))
+
+    split = make(_native.FORM_SPLIT)
+    T, M = ctypes.c_int(), ctypes.c_int()
+    L.lsamd_get_tile(_native.FORM_SPLIT, _native.STANDARD, _native.F64, ctypes.byref(T), ctypes.byref(M))
+    ms_split, kms_split = timed(split, args.steps, args.warmup)
+    log(f"split: {ms_split:.3f} ms/step, sweep kernel {kms_split:.3f} ms, tile {T.value}x{M.value}")
+
+    plain = make(_native.FORM_PLAIN)
+    ms_plain, kms_plain = timed(plain, args.fp64_steps, 1)
+    log(f"fp64 plain: {ms_plain:.3f} ms/step, sweep kernel {kms_plain:.3f} ms")
+
+    # full-size accuracy of the split form against the FP64 form (all bins)
+    ps = split.results()
+    pp = plain.results()
+    a, b = ps["pgram"], pp["pgram"]
+    pmax = b.max(dim=1, keepdim=True).values
+    mask = b >= 1e-3 * pmax
+    rel = ((a - b).abs() / b.abs()).where(mask, torch.zeros_like(a))
+    max_rel = float(rel.max())
+    max_abs = float(((a - b).abs() / pmax).max())
+    same_idx = int((ps["index"] == pp["index"]).sum())
+    sum_split = float(ps["period"].sum())
+    sum_plain = float(pp["period"].sum())
+    del a, b, rel, mask
+    log(f"accuracy: max rel {max_rel:.3e} (P>=1e-3 Pmax), max |dP|/Pmax {max_abs:.3e}, "
+        f"argmax identical {same_idx}/{K}, sum of periods {sum_split:.6f} vs {sum_plain:.6f}")
+
+    value = terms / (ms_split / 1e3)
+    kernel_terms_s = terms / (kms_split / 1e3)
+    achieved = kernel_terms_s * 2.0 * CYCLES_PER_TERM / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_TFLOPS, 2),
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "k_sweep_split<std,f64,%dx%d>" % (T.value, M.value),
+                "kernel_ms": round(kms_split, 4),
+                "work_per_term": f"{CYCLES_PER_TERM:g} FP32 lane-cycles = {2*CYCLES_PER_TERM:g} flop-eq "
+                                 f"(mul + sin + cos at 1/{TRANS_CYCLES:g} rate + 5 FMA)",
+                "roofline_terms_per_s": PEAK_LANE_CYCLES / CYCLES_PER_TERM,
+                "pgram_write_GBps": round(K * nf * 8 / (kms_split / 1e3) / 1e9, 1)}
+
+    out = {
+        "metric": "freq*obs terms/s (FP32-split form) and objects/s, 1000-object x 1e6-freq standard batch",
+        "value": value, "unit": "terms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_split, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "standard L-S batch: 1000 objects, log-normal N (median 140, sigma 0.6, "
+                               "20..1000), 1e6 angular freqs 1.005..150.796 rad/d, FP64 inputs in HBM, "
+                               "f64 periodogram written + fused argmax",
+                   "objects": K, "rows": rows, "freqs": nf, "terms_per_step": terms,
+                   "form": "split (FP64 phase, FP32 sincos+sums, FP64 power)",
+                   "tile": f"{T.value}x{M.value}"},
+        "objects_per_s": K / (ms_split / 1e3),
+        "fp64_plain": {"value": terms / (ms_plain / 1e3), "ms_per_step": round(ms_plain, 3),
+                       "kernel_ms": round(kms_plain, 3),
+                       "split_speedup": round(ms_plain / ms_split, 2)},
+        "accuracy": {"max_rel_err_masked": max_rel, "mask": "P >= 1e-3 * P_max per object",
+                     "max_abs_err_over_pmax": max_abs, "argmax_identical": f"{same_idx}/{K}",
+                     "sum_periods_split": sum_split, "sum_periods_fp64": sum_plain},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(data, nf, args.cpu_objects, args.cpu_threads)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,957 @@
+// HIP kernels of the MI355X-native Lomb-Scargle sweep and the stream-ordered
+// device entry point lsamd_sweep().
+//
+// Work decomposition: one workgroup per (object, frequency chunk).  A chunk
+// is T*M consecutive frequency bins; thread tid owns bins
+// chunk*T*M + m*T + tid (m < M).  The object's observations are staged
+// through LDS T at a time and read back as wave-wide broadcasts; all sums
+// live in registers.  Per-object normalisation and the argmax/top-3 peak
+// search are fused: every workgroup writes its chunk's best three bins and a
+// small reduction kernel merges the chunks of each object.
+//
+// Two arithmetic forms:
+//   plain  - the reference's arithmetic (source/main.cu:1566-1599 standard,
+//            main.cu:1701-1777 generalized): DTYPE phase omega*t, accurate
+//            sin/cos, sequential DTYPE sums with no FMA contraction, the
+//            tau form of the power.  Reproduces the reference CPU path.
+//   split  - FP64 phase reduction (revolutions, rounded to nearest), FP32
+//            hardware sin/cos of the reduced phase, an FP32 rotation
+//            recurrence across the thread's M bins (angle step T bins),
+//            FP32 partial sums over 64 observations folded into FP64
+//            accumulators, and the tau-free FP64 power
+//              standard:    P = (SS XC^2 - 2 CS XC XS + CC XS^2) / (2 (CC SS - CS^2))
+//              generalized: P = (SS YC^2 - 2 CS YC YS + CC YS^2) / ((CC SS - CS^2) YY)
+//            (centred weighted sums), equal to the reference's tau form
+//            because the 2x2 least-squares projection is rotation invariant.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "ls_internal.h"
+
+namespace lsamd {
+
+namespace {
+
+constexpr int kFlush = 64;     // observations per FP32 partial sum (split form)
+constexpr int kPeaks = 3;      // peaks tracked per object (top-1 and top-3)
+constexpr int kGen = 8;        // doubles per object of split-generalized constants
+constexpr double kInv2Pi = 0.159154943091895335768883763372514362;  // 1/(2 pi)
+
+// ---------------------------------------------------------------------------
+// Peak lists: best kPeaks (power, bin) pairs, ordered by power descending and
+// bin ascending.  Initialised to (0, 0): a bin enters only with power > 0,
+// and an equal power never displaces an earlier bin -- the reference's
+// "first index of the strict maximum, starting from 0" (main.cu:1140-1149,
+// 1545-1556) and its three-scan top-3 (main.cu:1165-1219).  NaN never enters.
+// ---------------------------------------------------------------------------
+struct Peaks {
+    double p[kPeaks];
+    uint32_t i[kPeaks];
+};
+
+__device__ __forceinline__ bool better(double p1, uint32_t i1, double p2, uint32_t i2) {
+    return p1 > p2 || (p1 == p2 && i1 < i2);
+}
+
+__device__ __forceinline__ void peaks_init(Peaks &L) {
+#pragma unroll
+    for (int r = 0; r < kPeaks; ++r) {
+        L.p[r] = 0.0;
+        L.i[r] = 0u;
+    }
+}
+
+// Insertion by compare-and-swap down the list (static indices only).
+__device__ __forceinline__ void peaks_insert(Peaks &L, double p, uint32_t i) {
+#pragma unroll
+    for (int r = 0; r < kPeaks; ++r) {
+        if (better(p, i, L.p[r], L.i[r])) {
+            const double tp = L.p[r];
+            const uint32_t ti = L.i[r];
+            L.p[r] = p;
+            L.i[r] = i;
+            p = tp;
+            i = ti;
+        }
+    }
+}
+
+// Merge the lists of a whole workgroup; the result is valid in thread 0.
+template <int T>
+__device__ __forceinline__ void block_peaks(Peaks &L, double (*s_p)[kPeaks],
+                                            uint32_t (*s_i)[kPeaks]) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        double op[kPeaks];
+        uint32_t oi[kPeaks];
+#pragma unroll
+        for (int r = 0; r < kPeaks; ++r) {
+            op[r] = __shfl_xor(L.p[r], off, 64);
+            oi[r] = __shfl_xor(L.i[r], off, 64);
+        }
+#pragma unroll
+        for (int r = 0; r < kPeaks; ++r) peaks_insert(L, op[r], oi[r]);
+    }
+    if constexpr (T > 64) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r < kPeaks; ++r) {
+                s_p[wave][r] = L.p[r];
+                s_i[wave][r] = L.i[r];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < T / 64; ++w)
+#pragma unroll
+                for (int r = 0; r < kPeaks; ++r) peaks_insert(L, s_p[w][r], s_i[w][r]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reference-arithmetic helpers (plain form)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void sincos_r(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ __forceinline__ void sincos_r(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ __forceinline__ double atan2_r(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ float atan2_r(float y, float x) { return atan2f(y, x); }
+__device__ __forceinline__ double sqrt_r(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_r(float x) { return sqrtf(x); }
+
+// ---------------------------------------------------------------------------
+// Kernel arguments
+// ---------------------------------------------------------------------------
+struct StatsArgs {
+    const uint64_t *offsets;
+    const void *y, *dy;
+    uint32_t K;
+    int normalize;       // scale = 2/(N sigma^2)  (main.cu:1295-1352)
+    int precenter;       // plain generalized: shift = sum(w y)/sum(w), w=1/sqrt(dy)
+    int split_gen;       // split generalized constants
+    double *obj_shift, *obj_scale, *obj_gen;
+};
+
+struct RecArgs {
+    const uint64_t *offsets;
+    const void *t, *y, *dy;
+    const double *obj_shift, *obj_gen;
+    void *rec_y, *rec_w;       // plain
+    double *rec_t;             // split
+    float4 *rec_a;             // split
+    double step_rev;           // split: T * df (revolutions/day per T bins)
+    int precenter;
+};
+
+struct SweepArgs {
+    const uint64_t *offsets;
+    const uint32_t *order;
+    uint32_t K, nf, chunks;
+    uint64_t item_base;
+    // plain
+    const void *t, *y, *w;
+    double fmin_r, fstep_r;    // DTYPE values stored in double
+    // split
+    const double *rec_t;
+    const float4 *rec_a;
+    double fmin_rev, df_rev;   // revolutions/day
+    const double *obj_gen;
+    // common
+    const double *obj_scale;
+    void *pgram;
+    double *part_pow;
+    uint32_t *part_idx;
+    int want_peaks;
+};
+
+struct ReduceArgs {
+    uint32_t K, chunks, top_k;
+    const double *part_pow;
+    const uint32_t *part_idx;
+    uint32_t *peak_index;
+    void *peak_power, *peak_period;
+    double pmin, pstep;        // period of bin i = 2 pi / (pmin + i*pstep)
+};
+
+// ---------------------------------------------------------------------------
+// Per-object statistics: one thread per object, sums in index order so that
+// the values equal the CPU path's sequential sums.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_obj_stats(StatsArgs a) {
+#pragma clang fp contract(off)
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.K) return;
+    const uint64_t b = a.offsets[k], e = a.offsets[k + 1];
+    const R *y = static_cast<const R *>(a.y);
+    const R *dy = static_cast<const R *>(a.dy);
+    const double nd = (double)(e - b);
+    double scale = 1.0;
+    if (a.normalize) {  // computeStandardDevSquared, main.cu:1295-1317
+        double sum = 0.0;
+        for (uint64_t i = b; i < e; ++i) sum += (double)y[i];
+        const double mean = sum / (nd * 1.0);
+        double sum2 = 0.0;
+        for (uint64_t i = b; i < e; ++i) sum2 += ((double)y[i] - mean) * ((double)y[i] - mean);
+        const double sigma = sqrt(sum2 / (nd * 1.0));
+        scale = 2.0 / (nd * (sigma * sigma));
+    }
+    a.obj_scale[k] = scale;
+    if (a.precenter) {  // updateYerrorfactor, main.cu:1631-1662 (index order)
+        R sumw = 0, dot = 0;
+        for (uint64_t i = b; i < e; ++i) sumw += (R)(1.0 / (double)sqrt_r(dy[i]));
+        for (uint64_t i = b; i < e; ++i) dot += (R)(1.0 / (double)sqrt_r(dy[i])) * y[i];
+        a.obj_shift[k] = (double)(dot / sumw);
+    }
+    if (a.split_gen) {
+        // weights 1/dy^2 normalised to unit sum; y centred on its weighted
+        // mean (the generalized power does not depend on the shift)
+        double W = 0.0, WY = 0.0;
+        for (uint64_t i = b; i < e; ++i) {
+            const double wi = 1.0 / ((double)dy[i] * (double)dy[i]);
+            W += wi;
+            WY += wi * (double)y[i];
+        }
+        const double ybar = WY / W;
+        double Wf = 0.0, Ys = 0.0, YYs = 0.0;
+        for (uint64_t i = b; i < e; ++i) {
+            const double wi = 1.0 / ((double)dy[i] * (double)dy[i]) / W;
+            const double yc = (double)y[i] - ybar;
+            const float wn = (float)wi, wy = (float)(wi * yc);
+            Wf += (double)wn;
+            Ys += (double)wy;
+            YYs += (double)wn * yc * yc;
+        }
+        const double Yb = Ys / Wf;
+        a.obj_gen[kGen * k + 0] = Wf;
+        a.obj_gen[kGen * k + 1] = Yb;
+        a.obj_gen[kGen * k + 2] = YYs / Wf - Yb * Yb;
+        a.obj_gen[kGen * k + 3] = ybar;
+        a.obj_gen[kGen * k + 4] = W;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-observation records.  One workgroup per object.
+// ---------------------------------------------------------------------------
+template <typename R, bool GEN>
+__global__ void k_records_plain(RecArgs a) {
+#pragma clang fp contract(off)
+    const uint32_t k = blockIdx.x;
+    const uint64_t b = a.offsets[k], e = a.offsets[k + 1];
+    const R *y = static_cast<const R *>(a.y);
+    const R *dy = static_cast<const R *>(a.dy);
+    R *ry = static_cast<R *>(a.rec_y);
+    R *rw = static_cast<R *>(a.rec_w);
+    const R shift = a.precenter ? (R)a.obj_shift[k] : (R)0;
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        if (a.precenter) ry[i] = y[i] - shift;  // main.cu:1656
+        if (GEN) {
+            R w = (R)(1.0 / (double)dy[i]);  // main.cu:1713-1714
+            w *= w;
+            rw[i] = w;
+        }
+    }
+}
+
+template <typename R, bool GEN>
+__global__ void k_records_split(RecArgs a) {
+    const uint32_t k = blockIdx.x;
+    const uint64_t b = a.offsets[k], e = a.offsets[k + 1];
+    const R *t = static_cast<const R *>(a.t);
+    const R *y = static_cast<const R *>(a.y);
+    const R *dy = static_cast<const R *>(a.dy);
+    double W = 1.0, ybar = 0.0;
+    if (GEN) {  // the same W and ybar k_obj_stats used, so the float weights match
+        ybar = a.obj_gen[kGen * k + 3];
+        W = a.obj_gen[kGen * k + 4];
+    }
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const double ti = (double)t[i];
+        double ps = a.step_rev * ti;
+        ps -= rint(ps);
+        double sv, cv;
+        sincospi(2.0 * ps, &sv, &cv);
+        float4 r;
+        if (GEN) {
+            const double wi = 1.0 / ((double)dy[i] * (double)dy[i]) / W;
+            r.x = (float)(wi * ((double)y[i] - ybar));
+            r.y = (float)wi;
+        } else {
+            r.x = (float)y[i];
+            r.y = 1.0f;
+        }
+        r.z = (float)cv;
+        r.w = (float)sv;
+        a.rec_t[i] = ti;
+        a.rec_a[i] = r;
+    }
+}
+
+__device__ __forceinline__ void item_coords(const SweepArgs &a, uint32_t &obj, uint32_t &chunk) {
+    const uint64_t item = a.item_base + blockIdx.x;
+    const uint32_t pos = (uint32_t)(item / a.chunks);
+    chunk = (uint32_t)(item - (uint64_t)pos * a.chunks);
+    obj = a.order ? a.order[pos] : pos;
+}
+
+template <typename OUT, int T>
+__device__ __forceinline__ void write_partial(const SweepArgs &a, Peaks &L, uint32_t obj,
+                                              uint32_t chunk, double (*s_p)[kPeaks],
+                                              uint32_t (*s_i)[kPeaks]) {
+    block_peaks<T>(L, s_p, s_i);
+    if (threadIdx.x == 0) {
+        const uint64_t o = ((uint64_t)obj * a.chunks + chunk) * kPeaks;
+#pragma unroll
+        for (int r = 0; r < kPeaks; ++r) {
+            a.part_pow[o + r] = L.p[r];
+            a.part_idx[o + r] = L.i[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain form: the reference's arithmetic, standard variant
+// (lombscarglecpuinnerloop, main.cu:1566-1599).
+// ---------------------------------------------------------------------------
+template <typename R, int T, int M>
+__global__ __launch_bounds__(T) void k_sweep_plain_std(SweepArgs a) {
+#pragma clang fp contract(off)
+    __shared__ R s_t[T], s_y[T];
+    __shared__ double s_pp[T / 64][kPeaks];
+    __shared__ uint32_t s_pi[T / 64][kPeaks];
+    uint32_t obj, chunk;
+    item_coords(a, obj, chunk);
+    const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
+    const R *gt = static_cast<const R *>(a.t);
+    const R *gy = static_cast<const R *>(a.y);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t kbase = chunk * (uint32_t)(T * M);
+    const R fmin = (R)a.fmin_r, fstep = (R)a.fstep_r;
+    R om[M], xc[M], xs[M], cc[M], ss[M], cs[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        om[m] = fmin + (fstep * (R)(kbase + m * T + tid));  // main.cu:1611
+        xc[m] = xs[m] = cc[m] = ss[m] = cs[m] = (R)0;
+    }
+    for (uint64_t b0 = beg; b0 < end; b0 += T) {
+        const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+        __syncthreads();
+        if (tid < cnt) {
+            s_t[tid] = gt[b0 + tid];
+            s_y[tid] = gy[b0 + tid];
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const R tj = s_t[j], yj = s_y[j];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                R s, c;
+                sincos_r(om[m] * tj, &s, &c);
+                xc[m] = xc[m] + yj * c;
+                xs[m] = xs[m] + yj * s;
+                cc[m] = cc[m] + c * c;
+                ss[m] = ss[m] + s * s;
+                cs[m] = cs[m] + c * s;
+            }
+        }
+    }
+    const double scale = a.obj_scale[obj];
+    R *pg = static_cast<R *>(a.pgram);
+    Peaks L;
+    peaks_init(L);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const uint32_t kf = kbase + m * T + tid;
+        const R omega = om[m];
+        const R tau = (R)(atan2(2.0 * (double)cs[m], (double)(cc[m] - ss[m])) / (2.0 * (double)omega));
+        R s_tau, c_tau;
+        sincos_r(omega * tau, &s_tau, &c_tau);
+        const R c_tau2 = c_tau * c_tau, s_tau2 = s_tau * s_tau;
+        const R cs_tau = (R)(2.0 * (double)c_tau * (double)s_tau);
+        const R p1 = c_tau * xc[m] + s_tau * xs[m];
+        const R p2 = c_tau * xs[m] - s_tau * xc[m];
+        const R q = (p1 * p1) / (c_tau2 * cc[m] + cs_tau * cs[m] + s_tau2 * ss[m]) +
+                    (p2 * p2) / (c_tau2 * ss[m] - cs_tau * cs[m] + s_tau2 * cc[m]);
+        R pw = (R)(0.5 * (double)q);
+        pw = (R)((double)pw * scale);  // main.cu:1335 / 1350
+        if (kf < a.nf) {
+            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+            peaks_insert(L, (double)pw, kf);
+        }
+    }
+    if (a.want_peaks) write_partial<R, T>(a, L, obj, chunk, s_pp, s_pi);
+}
+
+// ---------------------------------------------------------------------------
+// Plain form, generalized variant: two passes, tau then the tau-shifted sums
+// (lombscarglecpuinnerloopAstroPy, main.cu:1701-1777).
+// ---------------------------------------------------------------------------
+template <typename R, int T, int M>
+__global__ __launch_bounds__(T) void k_sweep_plain_gen(SweepArgs a) {
+#pragma clang fp contract(off)
+    __shared__ R s_t[T], s_y[T], s_w[T];
+    __shared__ double s_pp[T / 64][kPeaks];
+    __shared__ uint32_t s_pi[T / 64][kPeaks];
+    uint32_t obj, chunk;
+    item_coords(a, obj, chunk);
+    const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
+    const R *gt = static_cast<const R *>(a.t);
+    const R *gy = static_cast<const R *>(a.y);
+    const R *gw = static_cast<const R *>(a.w);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t kbase = chunk * (uint32_t)(T * M);
+    const R fmin = (R)a.fmin_r, fstep = (R)a.fstep_r;
+    R om[M], wsum[M], S[M], C[M], S2[M], C2[M], tau[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        om[m] = fmin + (fstep * (R)(kbase + m * T + tid));
+        wsum[m] = S[m] = C[m] = S2[m] = C2[m] = (R)0;
+    }
+    // pass 1: tau
+    for (uint64_t b0 = beg; b0 < end; b0 += T) {
+        const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+        __syncthreads();
+        if (tid < cnt) {
+            s_t[tid] = gt[b0 + tid];
+            s_w[tid] = gw[b0 + tid];
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const R tj = s_t[j], w = s_w[j];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                R s, c;
+                wsum[m] += w;
+                sincos_r(om[m] * tj, &s, &c);
+                S[m] += w * s;
+                C[m] += w * c;
+                S2[m] = (R)((double)S2[m] + 2.0 * (double)w * (double)s * (double)c);
+                C2[m] = (R)((double)C2[m] + ((double)w - 2.0 * (double)w * (double)s * (double)s));
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        S2[m] /= wsum[m];
+        C2[m] /= wsum[m];
+        S[m] /= wsum[m];
+        C[m] /= wsum[m];
+        S2[m] = (R)((double)S2[m] - 2.0 * (double)S[m] * (double)C[m]);
+        C2[m] -= (C[m] * C[m] - S[m] * S[m]);
+        tau[m] = (R)(0.5 * (double)atan2_r(S2[m], C2[m]) / (double)om[m]);
+    }
+    // pass 2: the tau-shifted sums
+    R Y[M], YY[M], Ct[M], St[M], YC[M], YS[M], CC[M], SS[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) Y[m] = YY[m] = Ct[m] = St[m] = YC[m] = YS[m] = CC[m] = SS[m] = (R)0;
+    for (uint64_t b0 = beg; b0 < end; b0 += T) {
+        const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+        __syncthreads();
+        if (tid < cnt) {
+            s_t[tid] = gt[b0 + tid];
+            s_y[tid] = gy[b0 + tid];
+            s_w[tid] = gw[b0 + tid];
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const R tj = s_t[j], yj = s_y[j], w = s_w[j];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                R s, c;
+                sincos_r(om[m] * (tj - tau[m]), &s, &c);
+                Y[m] += w * yj;
+                YY[m] += w * yj * yj;
+                Ct[m] += w * c;
+                St[m] += w * s;
+                YC[m] += w * yj * c;
+                YS[m] += w * yj * s;
+                CC[m] += w * c * c;
+                SS[m] += w * s * s;
+            }
+        }
+    }
+    R *pg = static_cast<R *>(a.pgram);
+    Peaks L;
+    peaks_init(L);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const uint32_t kf = kbase + m * T + tid;
+        const R ws = wsum[m];
+        R y_ = Y[m] / ws, yy = YY[m] / ws, ct = Ct[m] / ws, st = St[m] / ws;
+        R yc = YC[m] / ws, ys = YS[m] / ws, ccv = CC[m] / ws, ssv = SS[m] / ws;
+        yc -= y_ * ct;
+        ys -= y_ * st;
+        ccv -= ct * ct;
+        ssv -= st * st;
+        yy -= y_ * y_;
+        const R pw = (yc * yc / ccv + ys * ys / ssv) / yy;
+        if (kf < a.nf) {
+            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+            peaks_insert(L, (double)pw, kf);
+        }
+    }
+    if (a.want_peaks) write_partial<R, T>(a, L, obj, chunk, s_pp, s_pi);
+}
+
+// ---------------------------------------------------------------------------
+// Split form.  Per observation and thread: one FP64 multiply and round for
+// the seed phase (revolutions), one hardware sin/cos pair, then M bins by
+// rotation with the observation's precomputed step (cos, sin of T bins).
+// ---------------------------------------------------------------------------
+template <bool GEN, typename OUT, int T, int M>
+__global__ __launch_bounds__(T) void k_sweep_split(SweepArgs a) {
+    constexpr int NS = GEN ? 7 : 5;
+    __shared__ double s_t[T];
+    __shared__ float4 s_a[T];
+    __shared__ double s_pp[T / 64][kPeaks];
+    __shared__ uint32_t s_pi[T / 64][kPeaks];
+    uint32_t obj, chunk;
+    item_coords(a, obj, chunk);
+    const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t kbase = chunk * (uint32_t)(T * M);
+    const double fseed = a.fmin_rev + (double)(kbase + tid) * a.df_rev;
+
+    double acc[NS][M];
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
+
+    for (uint64_t b0 = beg; b0 < end; b0 += T) {
+        const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+        __syncthreads();
+        if (tid < cnt) {
+            s_t[tid] = a.rec_t[b0 + tid];
+            s_a[tid] = a.rec_a[b0 + tid];
+        }
+        __syncthreads();
+        for (uint32_t j0 = 0; j0 < cnt; j0 += kFlush) {
+            const uint32_t j1 = (j0 + kFlush < cnt) ? j0 + kFlush : cnt;
+            float p[NS][M];
+#pragma unroll
+            for (int q = 0; q < NS; ++q)
+#pragma unroll
+                for (int m = 0; m < M; ++m) p[q][m] = 0.0f;
+            for (uint32_t j = j0; j < j1; ++j) {
+                const double tj = s_t[j];
+                const float4 r = s_a[j];
+                double ph = fseed * tj;
+                ph -= __builtin_rint(ph);
+                const float phf = (float)ph;
+                float c = __builtin_amdgcn_cosf(phf);
+                float s = __builtin_amdgcn_sinf(phf);
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    if constexpr (GEN) {
+                        const float wc = r.y * c, ws = r.y * s;
+                        p[0][m] += wc;
+                        p[1][m] += ws;
+                        p[2][m] = fmaf(r.x, c, p[2][m]);
+                        p[3][m] = fmaf(r.x, s, p[3][m]);
+                        p[4][m] = fmaf(wc, c, p[4][m]);
+                        p[5][m] = fmaf(ws, s, p[5][m]);
+                        p[6][m] = fmaf(wc, s, p[6][m]);
+                    } else {
+                        p[0][m] = fmaf(r.x, c, p[0][m]);
+                        p[1][m] = fmaf(r.x, s, p[1][m]);
+                        p[2][m] = fmaf(c, c, p[2][m]);
+                        p[3][m] = fmaf(s, s, p[3][m]);
+                        p[4][m] = fmaf(c, s, p[4][m]);
+                    }
+                    if (m + 1 < M) {
+                        const float cn = fmaf(c, r.z, -s * r.w);
+                        const float sn = fmaf(s, r.z, c * r.w);
+                        c = cn;
+                        s = sn;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NS; ++q)
+#pragma unroll
+                for (int m = 0; m < M; ++m) acc[q][m] += (double)p[q][m];
+        }
+    }
+
+    const double scale = a.obj_scale[obj];
+    double Wf = 1.0, Yb = 0.0, YY = 1.0;
+    if constexpr (GEN) {
+        Wf = a.obj_gen[kGen * obj + 0];
+        Yb = a.obj_gen[kGen * obj + 1];
+        YY = a.obj_gen[kGen * obj + 2];
+    }
+    OUT *pg = static_cast<OUT *>(a.pgram);
+    Peaks L;
+    peaks_init(L);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const uint32_t kf = kbase + m * T + tid;
+        double P;
+        if constexpr (GEN) {
+            const double C = acc[0][m] / Wf, S = acc[1][m] / Wf;
+            const double YC = acc[2][m] / Wf - Yb * C, YS = acc[3][m] / Wf - Yb * S;
+            const double CC = acc[4][m] / Wf - C * C, SS = acc[5][m] / Wf - S * S;
+            const double CS = acc[6][m] / Wf - C * S;
+            const double D = CC * SS - CS * CS;
+            P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
+        } else {
+            const double XC = acc[0][m], XS = acc[1][m];
+            const double CC = acc[2][m], SS = acc[3][m], CS = acc[4][m];
+            const double D = CC * SS - CS * CS;
+            P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+        }
+        const OUT pw = (OUT)(P * scale);
+        if (kf < a.nf) {
+            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+            peaks_insert(L, (double)pw, kf);
+        }
+    }
+    if (a.want_peaks) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+}
+
+// ---------------------------------------------------------------------------
+// Merge the chunk partials of each object; period of bin i per the GPU host
+// path's formula (main.cu:1152).
+// ---------------------------------------------------------------------------
+template <typename OUT>
+__global__ void k_reduce_peaks(ReduceArgs a) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.K) return;
+    Peaks L;
+    peaks_init(L);
+    const uint64_t base = (uint64_t)k * a.chunks * kPeaks;
+    for (uint32_t c = 0; c < a.chunks; ++c)
+#pragma unroll
+        for (int r = 0; r < kPeaks; ++r)
+            peaks_insert(L, a.part_pow[base + (uint64_t)c * kPeaks + r],
+                         a.part_idx[base + (uint64_t)c * kPeaks + r]);
+    OUT *pp = static_cast<OUT *>(a.peak_power);
+    OUT *pe = static_cast<OUT *>(a.peak_period);
+    for (uint32_t r = 0; r < a.top_k && r < (uint32_t)kPeaks; ++r) {
+        const uint64_t o = (uint64_t)k * a.top_k + r;
+        if (a.peak_index) a.peak_index[o] = L.i[r];
+        if (pp) pp[o] = (OUT)L.p[r];
+        if (pe) pe[o] = (OUT)((1.0 / (a.pmin + (double)L.i[r] * a.pstep)) * 2.0 * M_PI);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile registry
+// ---------------------------------------------------------------------------
+typedef void (*SweepFn)(SweepArgs);
+
+struct TileEntry {
+    int form, variant, dtype, T, M;
+    SweepFn fn;
+};
+
+#define LS_SPLIT(GEN, OUTT, DT, TT, MM) \
+    {LSAMD_FORM_SPLIT, GEN, DT, TT, MM, k_sweep_split<(GEN) != 0, OUTT, TT, MM>}
+#define LS_PLAIN_STD(R, DT, TT, MM) {LSAMD_FORM_PLAIN, 0, DT, TT, MM, k_sweep_plain_std<R, TT, MM>}
+#define LS_PLAIN_GEN(R, DT, TT, MM) {LSAMD_FORM_PLAIN, 1, DT, TT, MM, k_sweep_plain_gen<R, TT, MM>}
+
+const TileEntry kTiles[] = {
+    // split form, both output types
+    LS_SPLIT(0, float, LSAMD_F32, 256, 8), LS_SPLIT(0, double, LSAMD_F64, 256, 8),
+    LS_SPLIT(0, float, LSAMD_F32, 256, 4), LS_SPLIT(0, double, LSAMD_F64, 256, 4),
+    LS_SPLIT(0, float, LSAMD_F32, 128, 8), LS_SPLIT(0, double, LSAMD_F64, 128, 8),
+    LS_SPLIT(0, float, LSAMD_F32, 512, 8), LS_SPLIT(0, double, LSAMD_F64, 512, 8),
+    LS_SPLIT(1, float, LSAMD_F32, 256, 4), LS_SPLIT(1, double, LSAMD_F64, 256, 4),
+    LS_SPLIT(1, float, LSAMD_F32, 256, 8), LS_SPLIT(1, double, LSAMD_F64, 256, 8),
+    LS_SPLIT(1, float, LSAMD_F32, 128, 4), LS_SPLIT(1, double, LSAMD_F64, 128, 4),
+    // plain form
+    LS_PLAIN_STD(float, LSAMD_F32, 256, 1), LS_PLAIN_STD(double, LSAMD_F64, 256, 1),
+    LS_PLAIN_STD(float, LSAMD_F32, 256, 2), LS_PLAIN_STD(double, LSAMD_F64, 256, 2),
+    LS_PLAIN_GEN(float, LSAMD_F32, 256, 1), LS_PLAIN_GEN(double, LSAMD_F64, 256, 1),
+};
+
+const TileEntry *find_tile(int form, int variant, int dtype, int T, int M) {
+    for (const TileEntry &e : kTiles)
+        if (e.form == form && e.variant == variant && e.dtype == dtype && e.T == T && e.M == M)
+            return &e;
+    return nullptr;
+}
+
+// user overrides: [form][variant] -> (T, M); 0 = built-in
+int g_tile_T[2][2] = {{0, 0}, {0, 0}};
+int g_tile_M[2][2] = {{0, 0}, {0, 0}};
+
+Tile default_tile(int form, int variant) {
+    if (form == LSAMD_FORM_SPLIT) return variant ? Tile{256, 4} : Tile{256, 8};
+    return Tile{256, 1};
+}
+
+bool g_timing = false;
+double g_last_ms = -1.0;
+hipEvent_t g_ev[2] = {nullptr, nullptr};
+bool g_ev_pending = false;
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+Tile current_tile(int form, int variant, int dtype) {
+    (void)dtype;
+    if (form < 0 || form > 1 || variant < 0 || variant > 1) return Tile{0, 0};
+    if (g_tile_T[form][variant] > 0) return Tile{g_tile_T[form][variant], g_tile_M[form][variant]};
+    return default_tile(form, variant);
+}
+
+bool tile_supported(int form, int variant, int threads, int fpt) {
+    return find_tile(form, variant, LSAMD_F32, threads, fpt) != nullptr &&
+           find_tile(form, variant, LSAMD_F64, threads, fpt) != nullptr;
+}
+
+int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
+    *w = Workspace{};
+    const Tile tile = current_tile(a->form, a->variant, a->dtype);
+    w->tile = tile;
+    const uint64_t per_chunk = (uint64_t)tile.threads * tile.fpt;
+    w->chunks = (uint32_t)((a->num_freqs + per_chunk - 1) / per_chunk);
+    const size_t K = a->num_objects, n = a->num_obs;
+    const size_t rsz = a->dtype == LSAMD_F64 ? 8 : 4;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes);
+        return o;
+    };
+    w->obj_shift = take(8 * K);
+    w->obj_scale = take(8 * K);
+    w->obj_gen = take(8 * kGen * K);
+    if (a->form == LSAMD_FORM_PLAIN) {
+        w->rec_y = take(rsz * n);
+        w->rec_w = take(rsz * n);
+    } else {
+        w->rec_t = take(8 * n);
+        w->rec_a = take(16 * n);
+    }
+    const size_t parts = (size_t)K * w->chunks * kPeaks;
+    w->part_pow = take(8 * parts);
+    w->part_idx = take(4 * parts);
+    w->bytes = off + 256;
+    return LSAMD_OK;
+}
+
+}  // namespace lsamd
+
+using namespace lsamd;
+
+extern "C" {
+
+size_t lsamd_sweep_workspace_size(const lsamd_sweep_args *a) {
+    if (!a) return 0;
+    Workspace w;
+    plan_workspace(a, &w);
+    return w.bytes;
+}
+
+int lsamd_set_tile(int form, int variant, int threads, int fpt) {
+    if (form < 0 || form > 1 || variant < 0 || variant > 1)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_set_tile: bad form/variant");
+    if (threads == 0 || fpt == 0) {
+        g_tile_T[form][variant] = g_tile_M[form][variant] = 0;
+        return LSAMD_OK;
+    }
+    if (!tile_supported(form, variant, threads, fpt))
+        return set_error(LSAMD_ERR_INVALID, "lsamd_set_tile: tile shape " + std::to_string(threads) +
+                                                "x" + std::to_string(fpt) + " not compiled");
+    g_tile_T[form][variant] = threads;
+    g_tile_M[form][variant] = fpt;
+    return LSAMD_OK;
+}
+
+void lsamd_get_tile(int form, int variant, int dtype, int *threads, int *fpt) {
+    const Tile t = current_tile(form, variant, dtype);
+    if (threads) *threads = t.threads;
+    if (fpt) *fpt = t.fpt;
+}
+
+void lsamd_set_timing(int enabled) { g_timing = enabled != 0; }
+
+double lsamd_last_sweep_kernel_ms(void) {
+    if (g_ev_pending) {
+        float ms = -1.0f;
+        if (hipEventElapsedTime(&ms, g_ev[0], g_ev[1]) == hipSuccess) g_last_ms = ms;
+        g_ev_pending = false;
+    }
+    return g_last_ms;
+}
+
+#define LS_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return set_error(LSAMD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_bytes, void *stream_v) {
+    if (!a) return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: null args");
+    if (a->variant != LSAMD_STANDARD && a->variant != LSAMD_GENERALIZED)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: bad variant");
+    if (a->dtype != LSAMD_F32 && a->dtype != LSAMD_F64)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: bad dtype");
+    if (a->form != LSAMD_FORM_PLAIN && a->form != LSAMD_FORM_SPLIT)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: bad form");
+    if (a->num_objects == 0 || a->num_freqs == 0 || a->num_obs == 0)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: empty input (objects, frequencies and "
+                                            "observations must all be >= 1)");
+    if (!a->offsets || !a->t || !a->y)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: null device pointer");
+    if (a->variant == LSAMD_GENERALIZED && !a->dy)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: generalized variant needs dy");
+    if (a->top_k != 0 && a->top_k != 1 && a->top_k != 3)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: top_k must be 0, 1 or 3");
+    if (!(a->max_freq > a->min_freq))
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: max_freq must exceed min_freq");
+    if (a->pgram == nullptr && a->top_k == 0)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: nothing to compute (no pgram, top_k=0)");
+    Workspace w;
+    plan_workspace(a, &w);
+    if (!workspace || workspace_bytes < w.bytes)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: workspace too small (need " +
+                                                std::to_string(w.bytes) + " bytes)");
+    const TileEntry *te = find_tile(a->form, a->variant, a->dtype, w.tile.threads, w.tile.fpt);
+    if (!te) return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: no kernel for this tile");
+
+    hipStream_t st = static_cast<hipStream_t>(stream_v);
+    char *ws = static_cast<char *>(workspace);
+    ws = reinterpret_cast<char *>((reinterpret_cast<uintptr_t>(ws) + 255) & ~(uintptr_t)255);
+    double *obj_shift = reinterpret_cast<double *>(ws + w.obj_shift);
+    double *obj_scale = reinterpret_cast<double *>(ws + w.obj_scale);
+    double *obj_gen = reinterpret_cast<double *>(ws + w.obj_gen);
+    const bool f64 = a->dtype == LSAMD_F64;
+    const bool gen = a->variant == LSAMD_GENERALIZED;
+    const bool split = a->form == LSAMD_FORM_SPLIT;
+    const bool plain_pre = !split && gen && a->precenter;
+    const uint32_t K = a->num_objects;
+    const int T = w.tile.threads, M = w.tile.fpt;
+
+    // 1. per-object statistics
+    StatsArgs sa{a->offsets, a->y, a->dy, K, a->normalize && !gen, plain_pre ? 1 : 0,
+                 (split && gen) ? 1 : 0, obj_shift, obj_scale, obj_gen};
+    const dim3 sb(128), sg((K + 127) / 128);
+    if (f64)
+        hipLaunchKernelGGL(k_obj_stats<double>, sg, sb, 0, st, sa);
+    else
+        hipLaunchKernelGGL(k_obj_stats<float>, sg, sb, 0, st, sa);
+    LS_HIP(hipGetLastError());
+
+    // 2. per-observation records
+    RecArgs ra{};
+    ra.offsets = a->offsets;
+    ra.t = a->t;
+    ra.y = a->y;
+    ra.dy = a->dy;
+    ra.obj_shift = obj_shift;
+    ra.obj_gen = obj_gen;
+    ra.precenter = plain_pre ? 1 : 0;
+    const double dw = (a->max_freq - a->min_freq) / ((double)a->num_freqs * 1.0);
+    if (split) {
+        ra.rec_t = reinterpret_cast<double *>(ws + w.rec_t);
+        ra.rec_a = reinterpret_cast<float4 *>(ws + w.rec_a);
+        ra.step_rev = (double)T * dw * kInv2Pi;
+        void (*fn)(RecArgs) = f64 ? (gen ? k_records_split<double, true> : k_records_split<double, false>)
+                                  : (gen ? k_records_split<float, true> : k_records_split<float, false>);
+        hipLaunchKernelGGL(fn, dim3(K), dim3(256), 0, st, ra);
+        LS_HIP(hipGetLastError());
+    } else if (gen) {
+        ra.rec_y = ws + w.rec_y;
+        ra.rec_w = ws + w.rec_w;
+        if (f64)
+            hipLaunchKernelGGL((k_records_plain<double, true>), dim3(K), dim3(256), 0, st, ra);
+        else
+            hipLaunchKernelGGL((k_records_plain<float, true>), dim3(K), dim3(256), 0, st, ra);
+        LS_HIP(hipGetLastError());
+    }
+
+    // 3. the sweep
+    SweepArgs sw{};
+    sw.offsets = a->offsets;
+    sw.order = a->order;
+    sw.K = K;
+    sw.nf = a->num_freqs;
+    sw.chunks = w.chunks;
+    sw.t = a->t;
+    sw.y = plain_pre ? (const void *)(ws + w.rec_y) : a->y;
+    sw.w = (!split && gen) ? (const void *)(ws + w.rec_w) : nullptr;
+    if (f64) {
+        const double fmin = a->min_freq, fmax = a->max_freq;
+        sw.fmin_r = fmin;
+        sw.fstep_r = (fmax - fmin) / ((double)a->num_freqs * 1.0);
+    } else {
+        const float fmin = (float)a->min_freq, fmax = (float)a->max_freq;
+        sw.fmin_r = (double)fmin;
+        sw.fstep_r = (double)(float)((double)(fmax - fmin) / ((double)a->num_freqs * 1.0));
+    }
+    sw.rec_t = split ? reinterpret_cast<const double *>(ws + w.rec_t) : nullptr;
+    sw.rec_a = split ? reinterpret_cast<const float4 *>(ws + w.rec_a) : nullptr;
+    sw.fmin_rev = a->min_freq * kInv2Pi;
+    sw.df_rev = dw * kInv2Pi;
+    sw.obj_gen = obj_gen;
+    sw.obj_scale = obj_scale;
+    sw.pgram = a->pgram;
+    sw.part_pow = reinterpret_cast<double *>(ws + w.part_pow);
+    sw.part_idx = reinterpret_cast<uint32_t *>(ws + w.part_idx);
+    sw.want_peaks = a->top_k > 0 ? 1 : 0;
+
+    const uint64_t items = (uint64_t)K * w.chunks;
+    const uint64_t max_items = (0xFFFFFFFFull / (uint64_t)T) - 1;
+    if (g_timing) {
+        if (!g_ev[0]) {
+            LS_HIP(hipEventCreate(&g_ev[0]));
+            LS_HIP(hipEventCreate(&g_ev[1]));
+        }
+        LS_HIP(hipEventRecord(g_ev[0], st));
+    }
+    for (uint64_t base = 0; base < items; base += max_items) {
+        const uint64_t cnt = std::min<uint64_t>(max_items, items - base);
+        sw.item_base = base;
+        void *kargs[] = {&sw};
+        LS_HIP(hipLaunchKernel(reinterpret_cast<const void *>(te->fn), dim3((uint32_t)cnt), dim3(T),
+                               kargs, 0, st));
+    }
+    if (g_timing) {
+        LS_HIP(hipEventRecord(g_ev[1], st));
+        g_ev_pending = true;
+    }
+    (void)M;
+
+    // 4. peaks
+    if (a->top_k > 0) {
+        ReduceArgs rd{};
+        rd.K = K;
+        rd.chunks = w.chunks;
+        rd.top_k = a->top_k;
+        rd.part_pow = sw.part_pow;
+        rd.part_idx = sw.part_idx;
+        rd.peak_index = a->peak_index;
+        rd.peak_power = a->peak_power;
+        rd.peak_period = a->peak_period;
+        if (split || f64) {
+            rd.pmin = a->min_freq;
+            rd.pstep = dw;
+        } else {
+            const float fmin = (float)a->min_freq, fmax = (float)a->max_freq;
+            rd.pmin = (double)fmin;
+            rd.pstep = (double)(fmax - fmin) / ((double)a->num_freqs * 1.0);
+        }
+        const dim3 rb(128), rg((K + 127) / 128);
+        if (f64)
+            hipLaunchKernelGGL(k_reduce_peaks<double>, rg, rb, 0, st, rd);
+        else
+            hipLaunchKernelGGL(k_reduce_peaks<float>, rg, rb, 0, st, rd);
+        LS_HIP(hipGetLastError());
+    }
+    return LSAMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+"""The Lomb-Scargle sweep on device-resident torch tensors.
+
+PyTorch-ROCm only supplies device memory and the stream here; the work is
+done by ``lsamd_sweep`` of the C ABI (include/lombscargle_amd.h), called
+through ctypes with raw device pointers.  Nothing falls back to a CPU path:
+without a GPU or without the built library the calls raise.
+
+``SweepPlan`` allocates every buffer once (outputs and workspace) so that
+``run()`` only enqueues kernels: that is what the benchmark times.
+"""
+from __future__ import annotations
+
+from ctypes import byref, c_void_p
+
+import numpy as np
+import torch
+
+from . import _native
+
+_TORCH_DTYPE = {_native.F32: torch.float32, _native.F64: torch.float64}
+
+
+def _dtype_code(tdtype):
+    if tdtype == torch.float32:
+        return _native.F32
+    if tdtype == torch.float64:
+        return _native.F64
+    raise ValueError(f"unsupported dtype {tdtype}")
+
+
+def offsets_from_counts(counts, device):
+    c = torch.as_tensor(counts, dtype=torch.int64)
+    off = torch.zeros(c.numel() + 1, dtype=torch.int64)
+    off[1:] = torch.cumsum(c, 0)
+    return off.to(device)
+
+
+class SweepPlan:
+    """Pre-allocated sweep over K objects held as contiguous row ranges.
+
+    t, y, dy: 1-D float32/float64 CUDA tensors (dy only for the generalized
+    variant); offsets: int64 CUDA tensor of K+1 row offsets.  order (optional
+    int32/uint32 CUDA tensor of K object numbers) is the processing order;
+    by default objects are processed longest first.
+    """
+
+    def __init__(self, t, y, dy, offsets, *, min_freq, max_freq, num_freqs, variant,
+                 form=_native.FORM_SPLIT, normalize=True, precenter=True, pgram=True, top_k=1,
+                 order="longest_first", lib=None):
+        if not t.is_cuda:
+            raise RuntimeError("SweepPlan needs CUDA (HIP) tensors")
+        self.lib = lib or _native.core()
+        self.dtype = _dtype_code(t.dtype)
+        dev = t.device
+        self.t, self.y = t.contiguous(), y.contiguous()
+        self.dy = dy.contiguous() if dy is not None else None
+        self.offsets = offsets.to(device=dev, dtype=torch.int64).contiguous()
+        K = self.offsets.numel() - 1
+        self.K, self.nf = K, int(num_freqs)
+        if isinstance(order, str) and order == "longest_first":
+            lens = (self.offsets[1:] - self.offsets[:-1]).cpu()
+            order = torch.argsort(-lens, stable=True).to(torch.int32).to(dev)
+        self.order = order.to(device=dev, dtype=torch.int32).contiguous() if order is not None else None
+        td = _TORCH_DTYPE[self.dtype]
+        self.pgram = torch.empty(K * self.nf, dtype=td, device=dev) if pgram else None
+        self.top_k = int(top_k)
+        kk = max(self.top_k, 1)
+        self.peak_index = torch.zeros(K * kk, dtype=torch.int32, device=dev)
+        self.peak_power = torch.zeros(K * kk, dtype=td, device=dev)
+        self.peak_period = torch.zeros(K * kk, dtype=td, device=dev)
+        a = _native.SweepArgs()
+        a.variant, a.dtype, a.form = int(variant), self.dtype, int(form)
+        a.normalize, a.precenter = int(bool(normalize)), int(bool(precenter))
+        a.num_objects, a.num_freqs, a.top_k = K, self.nf, self.top_k
+        a.num_obs = self.t.numel()
+        a.min_freq, a.max_freq = float(min_freq), float(max_freq)
+        a.offsets = self.offsets.data_ptr()
+        a.order = self.order.data_ptr() if self.order is not None else None
+        a.t, a.y = self.t.data_ptr(), self.y.data_ptr()
+        a.dy = self.dy.data_ptr() if self.dy is not None else None
+        a.pgram = self.pgram.data_ptr() if self.pgram is not None else None
+        if self.top_k:
+            a.peak_index = self.peak_index.data_ptr()
+            a.peak_power = self.peak_power.data_ptr()
+            a.peak_period = self.peak_period.data_ptr()
+        self.args = a
+        ws = int(self.lib.lsamd_sweep_workspace_size(byref(a)))
+        self.workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
+
+    def run(self, stream=None):
+        """Enqueue the sweep on `stream` (default: torch's current stream)."""
+        s = stream if stream is not None else torch.cuda.current_stream(self.t.device)
+        rc = self.lib.lsamd_sweep(byref(self.args), c_void_p(self.workspace.data_ptr()),
+                                  self.workspace.numel(), c_void_p(s.cuda_stream))
+        _native.check(self.lib, rc)
+
+    def results(self):
+        kk = max(self.top_k, 1)
+        out = {
+            "index": self.peak_index.view(self.K, kk)[:, :self.top_k],
+            "power": self.peak_power.view(self.K, kk)[:, :self.top_k],
+            "period": self.peak_period.view(self.K, kk)[:, :self.top_k],
+        }
+        if self.pgram is not None:
+            out["pgram"] = self.pgram.view(self.K, self.nf)
+        return out
+
+
+def sweep(t, y, dy, offsets, **kw):
+    """One-shot sweep; returns the result dict of SweepPlan.results()."""
+    plan = SweepPlan(t, y, dy, offsets, **kw)
+    plan.run()
+    return plan.results()
+
+
+def numpy_to_device(obj_id, t, y, dy, dtype=np.float64, device="cuda"):
+    """Host observation arrays -> (t, y, dy, offsets) CUDA tensors, objects
+    as runs of obj_id (None = one object)."""
+    npt = np.dtype(dtype)
+    t = np.ascontiguousarray(t, npt)
+    n = t.size
+    if obj_id is None:
+        off = np.array([0, n], np.int64)
+    else:
+        o = np.ascontiguousarray(obj_id, np.uint32)
+        K = int(_native.core().lsamd_count_objects(_native.ptr(o), n))
+        off = np.zeros(K + 1, np.uint64)
+        _native.core().lsamd_object_offsets(_native.ptr(o), n, _native.ptr(off), None)
+        off = off.astype(np.int64)
+    tt = torch.from_numpy(t).to(device)
+    yy = torch.from_numpy(np.ascontiguousarray(y, npt)).to(device)
+    dd = None if dy is None else torch.from_numpy(np.ascontiguousarray(dy, npt)).to(device)
+    return tt, yy, dd, torch.from_numpy(off).to(device)

@@ -1,0 +1,179 @@
+/*
+ * lombscargle_amd.h -- C ABI of the MI355X-native Lomb-Scargle engine.
+ *
+ * Two layers:
+ *
+ *  1. The drop-in entry point LombScarglePy, exported by each of the four
+ *     libraries libpyls{noerror,error}{float,double}.so exactly as the
+ *     reference exports it (source/main.cu:255-256; library names and flags
+ *     from source/Makefile:71-91).  Define LSAMD_DTYPE to float or double
+ *     before including this header to get the prototype of the library you
+ *     link; it defaults to float, like the reference Python wrapper.
+ *
+ *  2. The lsamd_* extension API, exported by the same libraries and by
+ *     libgpulsamd.so: a stream-ordered sweep over device-resident inputs
+ *     (lsamd_sweep), a host-array batch call (lsamd_batch_host), per-object
+ *     peak (top-k) outputs without materialising the periodogram, and error
+ *     reporting instead of process exit.  The reference has no counterpart
+ *     for these beyond the GPU batch path they replace (source/main.cu:742-1280).
+ *
+ * Everything is plain C: pointers, sizes, enums.  Device pointers are HIP
+ * device addresses; `stream` is a hipStream_t passed as void*.
+ */
+#ifndef LOMBSCARGLE_AMD_H
+#define LOMBSCARGLE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifndef LSAMD_DTYPE
+#define LSAMD_DTYPE float
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Drop-in entry point.  Replaces LombScarglePy, source/main.cu:255-411.
+ *
+ *   objectId[sizeData]  contiguous runs of equal ids are objects
+ *                       (computeObjectRanges, main.cu:468-510)
+ *   timeX, magY, magDY  observations; magDY is ignored by the standard
+ *                       libraries; the error libraries pre-centre magY IN
+ *                       PLACE, as the reference does (main.cu:1631-1662)
+ *   minFreq, maxFreq    angular frequencies (rad/day); maxFreq is excluded:
+ *                       omega_i = min + i*(max-min)/freqToTest
+ *   MODE                1 GPU batch, 2 GPU single object, 4 CPU batch,
+ *                       5 CPU single object; other values do nothing
+ *   pgram               caller-allocated, K*freqToTest, object-major, fully
+ *                       overwritten; the standard libraries normalise it by
+ *                       2/(N sigma^2) like the reference Python builds
+ *
+ * GPU modes compute in the reference's own arithmetic ("plain" form) unless
+ * the environment variable LSAMD_FORM=split selects the FP32-split form.
+ * Errors are reported on stderr and through lsamd_last_error*(); the call
+ * returns instead of exiting the process (LSAMD_EXIT_ON_ERROR=1 restores
+ * the reference's exit).
+ * ---------------------------------------------------------------------- */
+void LombScarglePy(unsigned int *objectId, LSAMD_DTYPE *timeX, LSAMD_DTYPE *magY,
+                   LSAMD_DTYPE *magDY, unsigned int sizeData, double minFreq, double maxFreq,
+                   unsigned int freqToTest, int MODE, LSAMD_DTYPE *pgram);
+
+/* ------------------------------------------------------------------------
+ * Extension API
+ * ---------------------------------------------------------------------- */
+enum lsamd_variant { LSAMD_STANDARD = 0, LSAMD_GENERALIZED = 1 };
+enum lsamd_dtype { LSAMD_F32 = 0, LSAMD_F64 = 1 };
+enum lsamd_form {
+    /* the reference's arithmetic: DTYPE phases omega*t, libm-accurate
+     * sin/cos, sequential DTYPE sums, tau form; matches the reference CPU
+     * path of the same DTYPE */
+    LSAMD_FORM_PLAIN = 0,
+    /* FP64 phase reduction, FP32 sin/cos with an FP32 rotation recurrence
+     * across frequencies, FP32 chunk sums folded into FP64, tau-free FP64
+     * power: within 1e-5 of FP64 on the bins that carry power */
+    LSAMD_FORM_SPLIT = 1
+};
+enum lsamd_status {
+    LSAMD_OK = 0,
+    LSAMD_ERR_INVALID = 1, /* bad argument */
+    LSAMD_ERR_HIP = 2,     /* HIP runtime error */
+    LSAMD_ERR_NOMEM = 3,   /* allocation failed */
+    LSAMD_ERR_NODEVICE = 4 /* no GPU visible */
+};
+
+/* One sweep over device-resident, object-contiguous observations.  Covers
+ * the batch kernels (kernel.cu:368-631), the single-object kernels
+ * (kernel.cu:76-257; K = 1), the normalisation (main.cu:1319-1352), the
+ * generalized pre-centring (main.cu:1631-1662) and the argmax/top-3
+ * reductions (main.cu:1136-1219, kernel.cu:5-17), all on the device. */
+typedef struct lsamd_sweep_args {
+    int32_t variant;      /* enum lsamd_variant */
+    int32_t dtype;        /* enum lsamd_dtype: type of t, y, dy, pgram, peak_power, peak_period */
+    int32_t form;         /* enum lsamd_form */
+    int32_t normalize;    /* standard: scale powers by 2/(N sigma_pop^2) */
+    int32_t precenter;    /* generalized, plain form: subtract sum(w y)/sum(w), w=1/sqrt(dy),
+                           * per object before the sweep (set 0 if y is already centred) */
+    uint32_t num_objects; /* K >= 1 */
+    uint32_t num_freqs;   /* Nf >= 1 */
+    uint32_t top_k;       /* peaks per object to report: 0, 1 or 3 */
+    uint64_t num_obs;     /* total observations n = offsets[K] */
+    double min_freq;      /* angular (rad/day) */
+    double max_freq;      /* angular, excluded */
+    const uint64_t *offsets; /* device, K+1: object k holds rows [offsets[k], offsets[k+1]) */
+    const uint32_t *order;   /* device, K: processing order (longest first is best), or NULL */
+    const void *t;           /* device, n */
+    const void *y;           /* device, n (not modified) */
+    const void *dy;          /* device, n; generalized only */
+    void *pgram;             /* device, K*Nf, or NULL to skip the periodogram */
+    uint32_t *peak_index;    /* device, K*top_k, or NULL */
+    void *peak_power;        /* device, K*top_k, or NULL */
+    void *peak_period;       /* device, K*top_k (days), or NULL */
+} lsamd_sweep_args;
+
+/* Bytes of device workspace lsamd_sweep needs for these arguments. */
+size_t lsamd_sweep_workspace_size(const lsamd_sweep_args *args);
+
+/* Enqueue the sweep on `stream` (NULL = default stream).  The workspace
+ * must be device memory of at least lsamd_sweep_workspace_size() bytes and
+ * stay untouched until the sweep completes.  Does not synchronise and does
+ * not allocate. */
+int lsamd_sweep(const lsamd_sweep_args *args, void *workspace, size_t workspace_bytes,
+                void *stream);
+
+/* Host-array batch: segmentation by runs of objectId, device staging,
+ * lsamd_sweep, copy-back, all synchronous.  y is pre-centred IN PLACE for
+ * the generalized variant when precenter is set (reference behaviour).
+ * pgram (K*Nf host elements of dtype) may be NULL; peak arrays (K*top_k)
+ * may be NULL.  *num_objects_out (may be NULL) receives K. */
+int lsamd_batch_host(int variant, int dtype, int form, int normalize, int precenter,
+                     const uint32_t *objectId, const void *t, void *y, const void *dy,
+                     uint64_t n, double min_freq, double max_freq, uint32_t num_freqs,
+                     void *pgram, uint32_t top_k, uint32_t *peak_index, void *peak_power,
+                     void *peak_period, uint32_t *num_objects_out);
+
+/* Reference CPU modes 4 and 5 (source/main.cu:1358-1543) with the same
+ * arithmetic, OpenMP over objects (batch) or frequencies (single). */
+int lsamd_cpu_batch(int variant, int dtype, int normalize, const uint32_t *objectId,
+                    const void *t, void *y, const void *dy, uint64_t n, double min_freq,
+                    double max_freq, uint32_t num_freqs, void *pgram, int nthreads);
+int lsamd_cpu_single(int variant, int dtype, int normalize, const void *t, void *y,
+                     const void *dy, uint64_t n, double min_freq, double max_freq,
+                     uint32_t num_freqs, void *pgram, int nthreads);
+
+/* Number of runs of equal ids (objects) in objectId[n]. */
+uint32_t lsamd_count_objects(const uint32_t *objectId, uint64_t n);
+/* Fill offsets[K+1] (row ranges) and ids[K] (may be NULL) of the runs. */
+uint32_t lsamd_object_offsets(const uint32_t *objectId, uint64_t n, uint64_t *offsets,
+                              uint32_t *ids);
+
+/* Status of the last failed call on this thread (LSAMD_OK if none). */
+int lsamd_last_error_code(void);
+const char *lsamd_last_error(void);
+void lsamd_clear_error(void);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int lsamd_device_count(void);
+
+/* When enabled, lsamd_sweep brackets its dominant kernel with HIP events on
+ * its stream; lsamd_last_sweep_kernel_ms() returns that duration after the
+ * stream has been synchronised (negative if unavailable). */
+void lsamd_set_timing(int enabled);
+double lsamd_last_sweep_kernel_ms(void);
+
+/* Tile shape of the sweep kernel: threads per workgroup (64..1024, a
+ * multiple of 64) and frequencies per thread (1, 2, 4 or 8).  Zero restores
+ * the built-in choice.  Returns LSAMD_ERR_INVALID for shapes not compiled. */
+int lsamd_set_tile(int form, int variant, int threads, int freqs_per_thread);
+/* Tile shape lsamd_sweep would use now (for reports). */
+void lsamd_get_tile(int form, int variant, int dtype, int *threads, int *freqs_per_thread);
+
+/* Library version (major*10000 + minor*100 + patch). */
+int lsamd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOMBSCARGLE_AMD_H */

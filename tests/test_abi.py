@@ -1,0 +1,190 @@
+"""The C ABI on the CPU: every library loads and exports what
+include/lombscargle_amd.h declares; the drop-in LombScarglePy reproduces the
+reference's CPU modes 4/5 bit for bit; argument checking and error
+reporting work without a GPU.  No GPU compute is attempted here."""
+import os
+import re
+import subprocess
+import sys
+from ctypes import byref, c_void_p
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import ROOT, load_data
+from gpu_lomb_scargle_amd import _native
+from gpu_lomb_scargle_amd import lombscarglegpu as ls
+
+TWO_PI = 2.0 * np.pi
+HEADER = os.path.join(ROOT, "include", "lombscargle_amd.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b(?:void|int|size_t|double|uint32_t|const char \*)\s*\**\s*(\w+)\s*\(", src)
+    return sorted(set(names))
+
+
+def exported(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    return {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+
+
+def test_header_declares_expected_api():
+    names = declared_functions()
+    assert "LombScarglePy" in names
+    assert set(_native.EXPORTED) <= set(names)
+
+
+@pytest.mark.parametrize("lib", [_native.CORE_LIB] + sorted(_native.DROPIN_LIBS.values()))
+def test_library_loads_and_exports_every_declared_symbol(lib):
+    path = os.path.join(_native.LIB_DIR, lib)
+    assert os.path.exists(path), f"{path} not built"
+    _native.load(lib)
+    syms = exported(path)
+    want = set(declared_functions())
+    if lib == _native.CORE_LIB:
+        want.discard("LombScarglePy")
+    missing = want - syms
+    assert not missing, f"{lib} lacks {sorted(missing)}"
+
+
+def test_no_device_is_reported_not_crashed():
+    L = _native.core()
+    if L.lsamd_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    t = np.linspace(0, 10, 20)
+    with pytest.raises(_native.LsamdError) as ei:
+        _native.batch_host(None, t, np.sin(t), None, 1.0, 2.0, 10, _native.STANDARD)
+    assert ei.value.code == _native.ERR_NODEVICE
+
+
+def test_sweep_argument_checks():
+    L = _native.core()
+    a = _native.SweepArgs()
+    a.variant, a.dtype, a.form = 0, 0, 0
+    a.num_objects, a.num_freqs, a.num_obs, a.top_k = 0, 10, 10, 1
+    a.min_freq, a.max_freq = 1.0, 2.0
+    assert L.lsamd_sweep(byref(a), None, 0, None) == _native.ERR_INVALID
+    assert b"empty" in L.lsamd_last_error()
+    a.num_objects = 1
+    a.offsets = a.t = a.y = 1  # never dereferenced: rejected before any launch
+    a.top_k = 2
+    assert L.lsamd_sweep(byref(a), None, 0, None) == _native.ERR_INVALID
+    a.top_k = 1
+    a.max_freq = 0.5
+    assert L.lsamd_sweep(byref(a), None, 0, None) == _native.ERR_INVALID
+    a.max_freq = 2.0
+    need = L.lsamd_sweep_workspace_size(byref(a))
+    assert need > 0
+    assert L.lsamd_sweep(byref(a), c_void_p(4096), need - 1, None) == _native.ERR_INVALID
+    assert b"workspace" in L.lsamd_last_error()
+    a.variant = 1  # generalized without dy
+    assert L.lsamd_sweep(byref(a), c_void_p(4096), need, None) == _native.ERR_INVALID
+
+
+def test_tile_registry():
+    L = _native.core()
+    assert L.lsamd_set_tile(1, 0, 256, 4) == _native.OK
+    from ctypes import c_int
+    T, M = c_int(), c_int()
+    L.lsamd_get_tile(1, 0, 0, byref(T), byref(M))
+    assert (T.value, M.value) == (256, 4)
+    assert L.lsamd_set_tile(1, 0, 0, 0) == _native.OK
+    assert L.lsamd_set_tile(1, 0, 96, 3) == _native.ERR_INVALID
+
+
+def test_segmentation_helpers_match_oracle():
+    L = _native.core()
+    ids = np.array([3, 3, 1, 1, 1, 3, 8, 8, 2], np.uint32)
+    K = L.lsamd_count_objects(_native.ptr(ids), ids.size)
+    off = np.zeros(K + 1, np.uint64)
+    got_ids = np.zeros(K, np.uint32)
+    assert L.lsamd_object_offsets(_native.ptr(ids), ids.size, _native.ptr(off), _native.ptr(got_ids)) == K
+    lo, hi, oids = oracle.object_ranges(ids)
+    assert off[:-1].tolist() == lo.tolist() and (off[1:] - 1).tolist() == hi.tolist()
+    assert got_ids.tolist() == oids.tolist()
+
+
+# --------------------------------------------------------------------------
+# drop-in LombScarglePy, CPU modes, against the oracle (bit for bit)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", ["float", "double"])
+@pytest.mark.parametrize("error", [False, True])
+def test_dropin_cpu_batch_mode4_bitwise(dtype, error):
+    o, t, y, dy = load_data("SDSS_stripe82_band_z.txt")
+    sl = slice(0, 1500)
+    # a float64 magY is pre-centred in place by the error libraries: pass a copy
+    ids, per, pg = ls.lombscargle(o[sl], t[sl], y[sl].copy(), 0.1, 10.0, error, "CPU",
+                                  dy[sl] if error else None, 2500, dtype, False)
+    npt = np.float32 if dtype == "float" else np.float64
+    ref = oracle.batch(o[sl], t[sl], y[sl], dy[sl] if error else None, TWO_PI * 0.1, TWO_PI * 10.0,
+                       2500, error, npt)
+    assert pg.dtype == npt and pg.tobytes() == ref[0].tobytes()
+    assert ids.tolist() == oracle.object_ranges(o[sl])[2].tolist()
+    exp = 1.0 / (0.1 + (10.0 - 0.1) / 2500 * np.argmax(ref[0], axis=1))
+    assert np.array_equal(per, exp)
+
+
+@pytest.mark.parametrize("dtype", ["float", "double"])
+@pytest.mark.parametrize("error", [False, True])
+def test_dropin_cpu_single_mode5_bitwise(dtype, error):
+    o, t, y, dy = load_data("8205_normalized_with_error.txt")
+    ids, per, pg = ls.lombscargle(o, t, y.copy(), 0.1, 10.0, error, "CPU", dy if error else None, 1500,
+                                  dtype, False)
+    npt = np.float32 if dtype == "float" else np.float64
+    ref = oracle.single(t, y, dy if error else None, TWO_PI * 0.1, TWO_PI * 10.0, 1500, error, npt)
+    assert pg.shape == (1, 1500) and pg[0].tobytes() == ref[0].tobytes()
+
+
+def test_dropin_precentres_callers_float64_array_in_place():
+    """Reference side effect (main.cu:1631-1662 on the caller's buffer)."""
+    o, t, y, dy = load_data("8205_normalized_with_error.txt")
+    y64 = np.ascontiguousarray(y[:200], np.float64)
+    before = y64.copy()
+    ls.lombscargle(o[:200], t[:200], y64, 0.1, 10.0, True, "CPU", dy[:200], 50, "double", False)
+    assert np.array_equal(y64, oracle.precenter(before, dy[:200]))
+    y32 = np.ascontiguousarray(y[:200], np.float32)  # float input is converted, caller untouched
+    before32 = y32.copy()
+    ls.lombscargle(o[:200], t[:200], y32, 0.1, 10.0, True, "CPU", dy[:200], 50, "float", False)
+    assert np.array_equal(y32, before32)
+
+
+def test_dropin_unknown_mode_does_nothing():
+    lib = _native.load(_native.DROPIN_LIBS[(False, "double")])
+    from ctypes import c_double, c_int, c_uint
+    t = np.linspace(0, 10, 16)
+    y = np.sin(t)
+    ids = np.zeros(16, np.uint32)
+    pg = np.full(8, -7.0)
+    lib.LombScarglePy.argtypes = None
+    lib.LombScarglePy(_native.ptr(ids), _native.ptr(t), _native.ptr(y), _native.ptr(np.ones(16)),
+                      c_uint(16), c_double(1.0), c_double(2.0), c_uint(8), c_int(3), _native.ptr(pg))
+    assert np.all(pg == -7.0)
+
+
+def test_python_error_without_magdy_exits():
+    t = np.linspace(0, 10, 16)
+    with pytest.raises(SystemExit):
+        ls.lombscargle(np.zeros(16), t, np.sin(t), 0.1, 1.0, True, "CPU", None, 8, "double", False)
+
+
+def test_python_helpers():
+    start, end, uniq = ls.computeIndexRangesForEachObject(np.array(["a", "a", "b", "a", "a"]))
+    assert start.tolist() == [0, 2, 3] and end.tolist() == [1, 2, 4]
+    assert uniq.tolist() == ["a", "b", "a"]
+    assert ls.enumerateObjects(start, end).tolist() == [0, 0, 1, 2, 2]
+    a = np.arange(6, dtype=np.float32)[::2]
+    b = ls.convert_type(a, np.float32)
+    assert b.flags["C_CONTIGUOUS"] and b.dtype == np.float32
+
+
+def test_gpu_mode_without_gpu_raises_cleanly():
+    if _native.core().lsamd_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    t = np.linspace(0, 10, 16)
+    with pytest.raises(RuntimeError):
+        ls.lombscargle(np.zeros(16), t, np.sin(t), 0.1, 1.0, False, "GPU", None, 8, "double", False)

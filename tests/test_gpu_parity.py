@@ -1,0 +1,288 @@
+"""GPU parity: the HIP path, called through the C ABI, against the CPU
+oracle and the golden fixtures of the reference's own CPU build.
+
+Tolerances (stated per test):
+  * plain form, FP64: relative 1e-12 on every bin whose power is at least
+    1e-6 of the object's maximum; |dP| <= 1e-12 * P_max on all bins;
+  * plain form, FP32: relative 1e-5 on bins with P >= 1e-2 * P_max and
+    |dP| <= 1e-5 * P_max on all bins (FP32 sums of ~100-1000 terms with
+    one-ulp sin/cos differences; near-zero bins are ill-conditioned);
+  * split form vs the FP64 oracle: relative 1e-5 on bins with
+    P >= 1e-3 * P_max, |dP| <= 2e-6 * P_max on all bins;
+  * argmax: identical wherever the best two bins differ by more than the
+    tolerance; top-k indices exact against the reference rule applied to
+    the GPU's own periodogram.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import load_data
+from gpu_lomb_scargle_amd import _native, synth
+from gpu_lomb_scargle_amd import lombscarglegpu as ls
+
+pytestmark = pytest.mark.gpu
+TWO_PI = 2.0 * np.pi
+F64, F32 = np.float64, np.float32
+
+
+def rel_report(got, ref, mask_frac):
+    got = np.atleast_2d(got).astype(np.float64)
+    ref = np.atleast_2d(ref).astype(np.float64)
+    pmax = np.nanmax(ref, axis=1, keepdims=True)
+    mask = ref >= mask_frac * pmax
+    rel = np.abs(got - ref) / np.abs(ref)
+    absn = np.abs(got - ref) / pmax
+    return float(np.nanmax(np.where(mask, rel, 0.0))), float(np.nanmax(absn))
+
+
+def assert_close(got, ref, rtol, mask_frac, atol_pmax):
+    r, a = rel_report(got, ref, mask_frac)
+    print(f"max rel (P >= {mask_frac:g} Pmax) = {r:.3e}, max |dP|/Pmax = {a:.3e}")
+    assert r <= rtol and a <= atol_pmax, (r, a)
+
+
+def argmax_agrees(got, ref, tol):
+    got, ref = np.atleast_2d(got), np.atleast_2d(ref)
+    bad = 0
+    for g, r in zip(got, ref):
+        top2 = np.sort(r)[-2:]
+        if (top2[1] - top2[0]) > tol * top2[1] and np.argmax(g) != np.argmax(r):
+            bad += 1
+    return bad
+
+
+def first_strict_max(row):
+    best, idx = 0.0, 0
+    m = np.nanmax(row)
+    if m > 0:
+        idx = int(np.flatnonzero(row == m)[0])
+        best = row[idx]
+    return idx, best
+
+
+# --------------------------------------------------------------------------
+# drop-in LombScarglePy (GPU modes) against the oracle
+# --------------------------------------------------------------------------
+TOL = {"double": (1e-12, 1e-6, 1e-12), "float": (1e-5, 1e-2, 1e-5)}
+
+
+@pytest.mark.parametrize("dtype", ["double", "float"])
+@pytest.mark.parametrize("error", [False, True])
+def test_dropin_gpu_batch_mode1(dtype, error):
+    o, t, y, dy = load_data("SDSS_stripe82_band_z.txt")
+    ids, per, pg = ls.lombscargle(o, t, y.copy(), 0.1, 10.0, error, "GPU", dy if error else None,
+                                  3000, dtype, False)
+    npt = F32 if dtype == "float" else F64
+    ref = oracle.batch(o, t, y, dy if error else None, TWO_PI * 0.1, TWO_PI * 10.0, 3000, error,
+                       npt, nthreads=8)
+    assert pg.shape == ref[0].shape and pg.dtype == npt
+    assert_close(pg, ref[0], *TOL[dtype])
+    assert argmax_agrees(pg, ref[0], 10 * TOL[dtype][0]) == 0
+
+
+@pytest.mark.parametrize("dtype", ["double", "float"])
+@pytest.mark.parametrize("error", [False, True])
+@pytest.mark.parametrize("fname", ["8205_normalized_with_error.txt", "243_normalized_ztf_filter2.txt"])
+def test_dropin_gpu_single_mode2(dtype, error, fname):
+    o, t, y, dy = load_data(fname)
+    if dy is None:
+        if error:
+            pytest.skip("no dy column")
+    ids, per, pg = ls.lombscargle(o, t, y.copy(), 0.1, 10.0, error, "GPU", dy if error else None,
+                                  4000, dtype, False)
+    npt = F32 if dtype == "float" else F64
+    ref = oracle.single(t, y, dy if error else None, TWO_PI * 0.1, TWO_PI * 10.0, 4000, error, npt,
+                        nthreads=8)
+    assert_close(pg[0], ref[0], *TOL[dtype])
+
+
+@pytest.mark.parametrize("name,dtype", [("double", F64), ("float", F32)])
+def test_golden_fixtures_on_gpu(golden, name, dtype):
+    o, t, y, dy = load_data("8205_normalized_with_error.txt")
+    r = _native.batch_host(None, t, y, None, TWO_PI * 0.1, TWO_PI * 10.0, 10000, _native.STANDARD,
+                           dtype, _native.FORM_PLAIN)
+    assert_close(r["pgram"][0], golden[f"s8205_std_{name}"], *TOL[name])
+    r = _native.batch_host(None, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 10000, _native.GENERALIZED,
+                           dtype, _native.FORM_PLAIN)
+    assert_close(r["pgram"][0], golden[f"s8205_gen_{name}"], *TOL[name])
+    o, t, y, _ = load_data("243_normalized_ztf_filter2.txt")
+    r = _native.batch_host(None, t, y, None, 3.142, 150.796, 10000, _native.STANDARD, dtype,
+                           _native.FORM_PLAIN)
+    assert_close(r["pgram"][0], golden[f"z243_std_{name}"], *TOL[name])
+    # SDSS batch: values around each peak and at fixed bins, argmax per object
+    o, t, y, dy = load_data("SDSS_stripe82_band_z.txt")
+    for err in (False, True):
+        r = _native.batch_host(o, t, y, dy if err else None, TWO_PI * 0.1, TWO_PI * 10.0, 20000,
+                               _native.GENERALIZED if err else _native.STANDARD, dtype,
+                               _native.FORM_PLAIN)
+        key = f"sdss_{'gen' if err else 'std'}_{name}"
+        pg = r["pgram"]
+        got_win = np.take_along_axis(pg, golden[key + "_win_index"], 1)
+        assert_close(got_win, golden[key + "_win_values"], *TOL[name])
+        got_fix = pg[:, golden["sdss_fixed_index"]]
+        rr, aa = rel_report(got_fix, golden[key + "_fixed_values"], TOL[name][1])
+        assert rr <= TOL[name][0]
+        assert np.array_equal(r["index"][:, 0], golden[key + "_argmax"])
+
+
+# --------------------------------------------------------------------------
+# split form against the FP64 oracle
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("error", [False, True])
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_split_form_vs_fp64_oracle(golden, error, dtype):
+    tag = "gen" if error else "std"
+    o, t, y = golden[f"syn_{tag}_obj"], golden[f"syn_{tag}_t"], golden[f"syn_{tag}_y"]
+    dy = golden[f"syn_{tag}_dy"] if error else None
+    nf = 20000
+    # the FP64 oracle sees the same input values the split form gets
+    # (float32 storage rounds t by up to 6e-5 d at t ~ 2000 d)
+    t, y = t.astype(dtype).astype(F64), y.astype(dtype).astype(F64)
+    dy = None if dy is None else dy.astype(dtype).astype(F64)
+    ref = oracle.batch(o, t, y, dy, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, error, F64, nthreads=8)
+    r = _native.batch_host(o, t, y, dy, synth.OMEGA_MIN, synth.OMEGA_MAX, nf,
+                           _native.GENERALIZED if error else _native.STANDARD, dtype,
+                           _native.FORM_SPLIT)
+    assert_close(r["pgram"], ref[0], 1e-5, 1e-3, 2e-6)
+    assert argmax_agrees(r["pgram"], ref[0], 1e-5) == 0
+
+
+def test_split_form_reference_data_sdss_and_ztf():
+    o, t, y, dy = load_data("SDSS_stripe82_band_z.txt")
+    ref = oracle.batch(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 5000, True, F64, nthreads=8)
+    r = _native.batch_host(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 5000, _native.GENERALIZED,
+                           F64, _native.FORM_SPLIT)
+    assert_close(r["pgram"], ref[0], 1e-5, 1e-3, 2e-6)
+    o, t, y, _ = load_data("243_normalized_ztf_filter2.txt")  # JD-scale times
+    ref = oracle.single(t, y, None, 3.142, 150.796, 20000, False, F64, nthreads=8)
+    r = _native.batch_host(None, t, y, None, 3.142, 150.796, 20000, _native.STANDARD, F64,
+                           _native.FORM_SPLIT)
+    assert_close(r["pgram"][0], ref[0], 1e-5, 1e-3, 2e-6)
+
+
+# --------------------------------------------------------------------------
+# fused peaks (argmax / top-3) against the reference rule
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT])
+def test_peaks_follow_reference_rule(form):
+    b = synth.lognormal_batch(40, 140, 0.6, 20, 1000, 1800, False, 21)
+    nf = 7777
+    for dt in (F64, F32):
+        r = _native.batch_host(b["obj_id"], b["t"], b["y"], None, synth.OMEGA_MIN, synth.OMEGA_MAX,
+                               nf, _native.STANDARD, dt, form, top_k=3)
+        pg = r["pgram"]
+        per3, pw3, ix3 = oracle.top3(pg, synth.OMEGA_MIN, synth.OMEGA_MAX, dt)
+        assert np.array_equal(r["index"], ix3)
+        assert np.array_equal(r["power"], pw3)
+        if form == _native.FORM_PLAIN:  # DTYPE grid (main.cu:1152)
+            assert np.array_equal(r["period"], per3)
+        else:  # the split form's grid is FP64
+            step = (synth.OMEGA_MAX - synth.OMEGA_MIN) / nf
+            exp = (1.0 / (synth.OMEGA_MIN + ix3.astype(F64) * step) * 2.0 * np.pi).astype(dt)
+            assert np.allclose(r["period"], exp, rtol=2e-16 if dt == F64 else 1e-7)
+        r1 = _native.batch_host(b["obj_id"], b["t"], b["y"], None, synth.OMEGA_MIN,
+                                synth.OMEGA_MAX, nf, _native.STANDARD, dt, form, top_k=1, pgram=False)
+        assert np.array_equal(r1["index"][:, 0], ix3[:, 0])
+
+
+def test_peaks_ties_and_nonpositive():
+    # a constant series gives identical (degenerate) powers: the reference
+    # rule picks bin 0 whenever nothing is > 0
+    t = np.linspace(0.0, 30.0, 40)
+    y = np.zeros(40)
+    r = _native.batch_host(None, t, y, None, 1.0, 9.0, 500, _native.STANDARD, F64, _native.FORM_PLAIN,
+                           normalize=False, top_k=3)
+    assert r["index"][0].tolist() == [0, 0, 0]
+
+
+# --------------------------------------------------------------------------
+# edge cases: staging boundaries, ragged grids, tiny and degenerate objects
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT])
+@pytest.mark.parametrize("nf", [1, 3, 255, 2049, 4097])
+def test_ragged_grids_and_staging_boundaries(form, nf):
+    rng = np.random.default_rng(nf)
+    counts = [1, 2, 5, 255, 256, 257, 600, 64, 65]
+    ts, ys, ids = [], [], []
+    for k, n in enumerate(counts):
+        t = np.sort(rng.uniform(0, 500, n))
+        ts.append(t)
+        ys.append(np.sin(1.3 * t) + rng.normal(0, 0.3, n))
+        ids.append(np.full(n, k, np.uint32))
+    o, t, y = np.concatenate(ids), np.concatenate(ts), np.concatenate(ys)
+    ref = oracle.batch(o, t, y, None, 1.0, 30.0, nf, False, F64, nthreads=8)
+    r = _native.batch_host(o, t, y, None, 1.0, 30.0, nf, _native.STANDARD, F64, form)
+    pg = r["pgram"]
+    ok = np.array(counts) >= 3   # N<3: degenerate least squares, values not comparable
+    if form == _native.FORM_PLAIN:
+        assert_close(pg[ok], ref[0][ok], 1e-12, 1e-6, 1e-12)
+    else:
+        assert_close(pg[ok], ref[0][ok], 1e-5, 1e-3, 2e-6)
+
+
+def test_many_tiny_objects():
+    b = synth.lognormal_batch(3000, 8, 0.5, 5, 40, 1800, False, 5)
+    ref = oracle.batch(b["obj_id"], b["t"], b["y"], None, synth.OMEGA_MIN, synth.OMEGA_MAX, 300,
+                       False, F64, nthreads=8)
+    r = _native.batch_host(b["obj_id"], b["t"], b["y"], None, synth.OMEGA_MIN, synth.OMEGA_MAX, 300,
+                           _native.STANDARD, F64, _native.FORM_PLAIN)
+    assert_close(r["pgram"], ref[0], 1e-12, 1e-6, 1e-12)
+    assert np.array_equal(r["index"][:, 0], ref[3])
+
+
+def test_memory_budget_batching_is_exact():
+    b = synth.lognormal_batch(50, 100, 0.5, 20, 400, 1800, True, 9)
+    full = _native.batch_host(b["obj_id"], b["t"], b["y"], b["dy"], 1.0, 60.0, 3000,
+                              _native.GENERALIZED, F32, _native.FORM_SPLIT)
+    os.environ["LSAMD_PGRAM_BUDGET_BYTES"] = str(7 * 3000 * 4)  # 7 objects per batch
+    try:
+        part = _native.batch_host(b["obj_id"], b["t"], b["y"], b["dy"], 1.0, 60.0, 3000,
+                                  _native.GENERALIZED, F32, _native.FORM_SPLIT)
+    finally:
+        del os.environ["LSAMD_PGRAM_BUDGET_BYTES"]
+    assert np.array_equal(full["pgram"], part["pgram"])
+    assert np.array_equal(full["index"], part["index"])
+
+
+def test_identical_times_degenerate_like_reference():
+    t = np.full(10, 3.5)
+    y = np.linspace(-1, 1, 10)
+    ref = oracle.single(t, y, None, 1.0, 5.0, 50, False, F64, normalize=False)
+    r = _native.batch_host(None, t, y, None, 1.0, 5.0, 50, _native.STANDARD, F64, _native.FORM_PLAIN,
+                           normalize=False)
+    same = (np.isnan(ref[0]) & np.isnan(r["pgram"][0])) | (ref[0] == r["pgram"][0]) | \
+           (np.abs(ref[0] - r["pgram"][0]) <= 1e-12 * np.abs(ref[0]))
+    assert same.mean() > 0.9
+
+
+# --------------------------------------------------------------------------
+# device-tensor API (torch supplies memory and stream)
+# --------------------------------------------------------------------------
+def test_device_plan_matches_host_path():
+    torch = pytest.importorskip("torch")
+    from gpu_lomb_scargle_amd import device
+    b = synth.lognormal_batch(64, 140, 0.6, 20, 1000, 2000, True, 33)
+    host = _native.batch_host(b["obj_id"], b["t"], b["y"], b["dy"], synth.OMEGA_MIN,
+                              synth.OMEGA_MAX, 9000, _native.GENERALIZED, F32, _native.FORM_SPLIT,
+                              precenter=True, top_k=3)
+    # the host path pre-centred y in place (reference side effect); give the
+    # device plan the same magnitudes
+    t, y, dy, off = device.numpy_to_device(b["obj_id"], b["t"], host["y"], b["dy"], F32)
+    plan = device.SweepPlan(t, y, dy, off, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
+                            num_freqs=9000, variant=_native.GENERALIZED, form=_native.FORM_SPLIT,
+                            top_k=3)
+    plan.run()
+    torch.cuda.synchronize()
+    res = plan.results()
+    assert np.array_equal(res["pgram"].cpu().numpy(), host["pgram"])
+    assert np.array_equal(res["index"].cpu().numpy().astype(np.uint32), host["index"])
+    # peaks only (no periodogram): same peaks
+    plan2 = device.SweepPlan(t, y, dy, off, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
+                             num_freqs=9000, variant=_native.GENERALIZED, form=_native.FORM_SPLIT,
+                             top_k=3, pgram=False)
+    plan2.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(plan2.results()["index"].cpu().numpy(), res["index"].cpu().numpy())
