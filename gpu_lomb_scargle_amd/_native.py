@@ -25,7 +25,7 @@ DROPIN_LIBS = {
 
 STANDARD, GENERALIZED = 0, 1
 F32, F64 = 0, 1
-FORM_PLAIN, FORM_SPLIT = 0, 1
+FORM_PLAIN, FORM_SPLIT, FORM_SPLIT32 = 0, 1, 2
 OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_NODEVICE = 0, 1, 2, 3, 4
 
 # Every function include/lombscargle_amd.h declares (LombScarglePy only in the
@@ -94,10 +94,22 @@ def _declare(L):
 _cache = {}
 
 
+def _share_torch_runtime():
+    """torch-ROCm bundles its own libamdhip64.so.7 (same soname as ROCm's).
+    Whichever copy is loaded first serves the whole process, and torch only
+    initialises against its own, so when torch is installed it is imported
+    before this library is loaded; both then share torch's HIP runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(name=CORE_LIB, lib_dir=None):
     """Load one of the built libraries (raises NativeLibraryError if absent)."""
     path = os.path.join(lib_dir or LIB_DIR, name)
     if path not in _cache:
+        _share_torch_runtime()
         if not os.path.exists(path):
             raise NativeLibraryError(
                 f"{path} is not built; run __graft_entry__.build() or make -C gpu_lomb_scargle_amd/csrc")

@@ -44,6 +44,7 @@ constexpr const char *kDtypeName = sizeof(real_t) == 8 ? "double" : "float";
 int env_form() {
     const char *v = std::getenv("LSAMD_FORM");
     if (v && (std::strcmp(v, "split") == 0 || std::strcmp(v, "1") == 0)) return LSAMD_FORM_SPLIT;
+    if (v && (std::strcmp(v, "split32") == 0 || std::strcmp(v, "2") == 0)) return LSAMD_FORM_SPLIT32;
     return LSAMD_FORM_PLAIN;
 }
 
@@ -141,7 +142,9 @@ extern "C" void LombScarglePy(unsigned int *objectId, real_t *timeX, real_t *mag
                          "%g, Sum of periods: %g, Min/Max Freq: %g/%g,  Num tested freq: %u, MODE: %d, "
                          "ERROR/DTYPE/FORM/TILE: %d, %s, %s, %dx%d\n",
                          secs, sumPeriods, minFreq, maxFreq, freqToTest, MODE, LSAMD_ABI_ERROR,
-                         kDtypeName, form == LSAMD_FORM_SPLIT ? "split" : "plain", T, M);
+                         kDtypeName,
+                         form == LSAMD_FORM_SPLIT ? "split" : (form == LSAMD_FORM_SPLIT32 ? "split32" : "plain"),
+                         T, M);
             std::fclose(f);
         }
     }

@@ -38,6 +38,7 @@ struct Workspace {
     size_t rec_w = 0;       // DTYPE[n]   plain generalized: (1/dy)^2 in DTYPE
     size_t rec_t = 0;       // double[n]  split: t
     size_t rec_a = 0;       // float4[n]  split: {y or w*y, w, cos step, sin step}
+    size_t rec_r = 0;       // double2[n] split: {cos step, sin step}
     // per (object, chunk) peak partials
     size_t part_pow = 0;    // double[K*chunks*topk]
     size_t part_idx = 0;    // uint32[K*chunks*topk]

@@ -41,6 +41,7 @@ constexpr int kFlush = 64;     // observations per FP32 partial sum (split form)
 constexpr int kPeaks = 3;      // peaks tracked per object (top-1 and top-3)
 constexpr int kGen = 8;        // doubles per object of split-generalized constants
 constexpr double kInv2Pi = 0.159154943091895335768883763372514362;  // 1/(2 pi)
+constexpr double kTwoPi = 6.283185307179586476925286766559005768;   // 2 pi
 
 // ---------------------------------------------------------------------------
 // Peak lists: best kPeaks (power, bin) pairs, ordered by power descending and
@@ -118,10 +119,20 @@ __device__ __forceinline__ void block_peaks(Peaks &L, double (*s_p)[kPeaks],
 // ---------------------------------------------------------------------------
 // Reference-arithmetic helpers (plain form)
 // ---------------------------------------------------------------------------
+// The FP32 plain form evaluates sin, cos and atan2 of its float arguments in
+// FP64 and rounds once: correctly rounded float results (double rounding is
+// possible only within 2^-29 of a float midpoint).  The reference's float CPU
+// build calls glibc's sinf/cosf/atan2f, which are correctly rounded on most
+// but not all inputs, so this is the closest reproducible target.
 __device__ __forceinline__ void sincos_r(double x, double *s, double *c) { sincos(x, s, c); }
-__device__ __forceinline__ void sincos_r(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ __forceinline__ void sincos_r(float x, float *s, float *c) {
+    double sd, cd;
+    sincos((double)x, &sd, &cd);
+    *s = (float)sd;
+    *c = (float)cd;
+}
 __device__ __forceinline__ double atan2_r(double y, double x) { return atan2(y, x); }
-__device__ __forceinline__ float atan2_r(float y, float x) { return atan2f(y, x); }
+__device__ __forceinline__ float atan2_r(float y, float x) { return (float)atan2((double)y, (double)x); }
 __device__ __forceinline__ double sqrt_r(double x) { return sqrt(x); }
 __device__ __forceinline__ float sqrt_r(float x) { return sqrtf(x); }
 
@@ -144,7 +155,8 @@ struct RecArgs {
     const double *obj_shift, *obj_gen;
     void *rec_y, *rec_w;       // plain
     double *rec_t;             // split
-    float4 *rec_a;             // split
+    float4 *rec_a;             // split: {y or w*y, w, cos step, sin step} (FP32)
+    double2 *rec_r;            // split: {cos step, sin step} (FP64)
     double step_rev;           // split: T * df (revolutions/day per T bins)
     int precenter;
 };
@@ -160,6 +172,7 @@ struct SweepArgs {
     // split
     const double *rec_t;
     const float4 *rec_a;
+    const double2 *rec_r;
     double fmin_rev, df_rev;   // revolutions/day
     const double *obj_gen;
     // common
@@ -291,6 +304,7 @@ __global__ void k_records_split(RecArgs a) {
         r.w = (float)sv;
         a.rec_t[i] = ti;
         a.rec_a[i] = r;
+        a.rec_r[i] = make_double2(cv, sv);
     }
 }
 
@@ -501,12 +515,14 @@ __global__ __launch_bounds__(T) void k_sweep_plain_gen(SweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Split form.  Per observation and thread: one FP64 multiply and round for
-// the seed phase (revolutions), one hardware sin/cos pair, then M bins by
-// rotation with the observation's precomputed step (cos, sin of T bins).
+// Split form, FP32 sums ("split32").  Per observation and thread: one FP64
+// multiply and round for the seed phase (revolutions), one hardware sin/cos
+// pair, then M bins by FP32 rotation with the observation's precomputed step
+// (cos, sin of T bins); FP32 partial sums over 64 observations are folded
+// into FP64 accumulators.
 // ---------------------------------------------------------------------------
 template <bool GEN, typename OUT, int T, int M>
-__global__ __launch_bounds__(T) void k_sweep_split(SweepArgs a) {
+__global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
     constexpr int NS = GEN ? 7 : 5;
     __shared__ double s_t[T];
     __shared__ float4 s_a[T];
@@ -618,11 +634,131 @@ __global__ __launch_bounds__(T) void k_sweep_split(SweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Split form, FP64 rotation and sums ("split", the default fast form).  The
+// seed of each (thread, observation) is the hardware FP32 sin/cos of the
+// FP64-reduced phase, corrected to first order by the FP64 residual of that
+// phase's rounding to FP32; the M bins then follow by FP64 rotation with the
+// observation's FP64 step and accumulate directly in FP64.  On gfx950 FP64
+// FMA issues at the FP32 rate, so this costs ~10% more instructions than the
+// FP32-sum form and removes the FP32 summation error that dominates at
+// ill-conditioned (aliasing) frequencies.
+// ---------------------------------------------------------------------------
+template <bool GEN, typename OUT, int T, int M>
+__global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
+    constexpr int NS = GEN ? 7 : 5;
+    __shared__ double s_t[T];
+    __shared__ double2 s_r[T];
+    __shared__ float2 s_a[T];
+    __shared__ double s_pp[T / 64][kPeaks];
+    __shared__ uint32_t s_pi[T / 64][kPeaks];
+    uint32_t obj, chunk;
+    item_coords(a, obj, chunk);
+    const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t kbase = chunk * (uint32_t)(T * M);
+    const double fseed = a.fmin_rev + (double)(kbase + tid) * a.df_rev;
+
+    double acc[NS][M];
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
+
+    for (uint64_t b0 = beg; b0 < end; b0 += T) {
+        const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+        __syncthreads();
+        if (tid < cnt) {
+            s_t[tid] = a.rec_t[b0 + tid];
+            s_r[tid] = a.rec_r[b0 + tid];
+            const float4 r4 = a.rec_a[b0 + tid];
+            s_a[tid] = make_float2(r4.x, r4.y);
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const double tj = s_t[j];
+            const double2 st = s_r[j];
+            const float2 yw = s_a[j];
+            double ph = fseed * tj;
+            ph -= __builtin_rint(ph);
+            const float phf = (float)ph;
+            const float cf = __builtin_amdgcn_cosf(phf);
+            const float sf = __builtin_amdgcn_sinf(phf);
+            const double res = (ph - (double)phf) * kTwoPi;  // exact residual, radians
+            double c = fma(-res, (double)sf, (double)cf);
+            double s = fma(res, (double)cf, (double)sf);
+            const double yv = (double)yw.x;
+            const double wv = (double)yw.y;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if constexpr (GEN) {
+                    const double wc = wv * c, ws = wv * s;
+                    acc[0][m] += wc;
+                    acc[1][m] += ws;
+                    acc[2][m] = fma(yv, c, acc[2][m]);
+                    acc[3][m] = fma(yv, s, acc[3][m]);
+                    acc[4][m] = fma(wc, c, acc[4][m]);
+                    acc[5][m] = fma(ws, s, acc[5][m]);
+                    acc[6][m] = fma(wc, s, acc[6][m]);
+                } else {
+                    acc[0][m] = fma(yv, c, acc[0][m]);
+                    acc[1][m] = fma(yv, s, acc[1][m]);
+                    acc[2][m] = fma(c, c, acc[2][m]);
+                    acc[3][m] = fma(s, s, acc[3][m]);
+                    acc[4][m] = fma(c, s, acc[4][m]);
+                }
+                if (m + 1 < M) {
+                    const double cn = fma(c, st.x, -s * st.y);
+                    const double sn = fma(s, st.x, c * st.y);
+                    c = cn;
+                    s = sn;
+                }
+            }
+        }
+    }
+
+    const double scale = a.obj_scale[obj];
+    double Wf = 1.0, Yb = 0.0, YY = 1.0;
+    if constexpr (GEN) {
+        Wf = a.obj_gen[kGen * obj + 0];
+        Yb = a.obj_gen[kGen * obj + 1];
+        YY = a.obj_gen[kGen * obj + 2];
+    }
+    OUT *pg = static_cast<OUT *>(a.pgram);
+    Peaks L;
+    peaks_init(L);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const uint32_t kf = kbase + m * T + tid;
+        double P;
+        if constexpr (GEN) {
+            const double C = acc[0][m] / Wf, S = acc[1][m] / Wf;
+            const double YC = acc[2][m] / Wf - Yb * C, YS = acc[3][m] / Wf - Yb * S;
+            const double CC = acc[4][m] / Wf - C * C, SS = acc[5][m] / Wf - S * S;
+            const double CS = acc[6][m] / Wf - C * S;
+            const double D = CC * SS - CS * CS;
+            P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
+        } else {
+            const double XC = acc[0][m], XS = acc[1][m];
+            const double CC = acc[2][m], SS = acc[3][m], CS = acc[4][m];
+            const double D = CC * SS - CS * CS;
+            P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+        }
+        const OUT pw = (OUT)(P * scale);
+        if (kf < a.nf) {
+            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+            peaks_insert(L, (double)pw, kf);
+        }
+    }
+    if (a.want_peaks) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+}
+
+// ---------------------------------------------------------------------------
 // Merge the chunk partials of each object; period of bin i per the GPU host
 // path's formula (main.cu:1152).
 // ---------------------------------------------------------------------------
 template <typename OUT>
 __global__ void k_reduce_peaks(ReduceArgs a) {
+#pragma clang fp contract(off)
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.K) return;
     Peaks L;
@@ -653,20 +789,27 @@ struct TileEntry {
     SweepFn fn;
 };
 
+#define LS_SPLIT32(GEN, OUTT, DT, TT, MM) \
+    {LSAMD_FORM_SPLIT32, GEN, DT, TT, MM, k_sweep_split32<(GEN) != 0, OUTT, TT, MM>}
 #define LS_SPLIT(GEN, OUTT, DT, TT, MM) \
-    {LSAMD_FORM_SPLIT, GEN, DT, TT, MM, k_sweep_split<(GEN) != 0, OUTT, TT, MM>}
+    {LSAMD_FORM_SPLIT, GEN, DT, TT, MM, k_sweep_split64<(GEN) != 0, OUTT, TT, MM>}
 #define LS_PLAIN_STD(R, DT, TT, MM) {LSAMD_FORM_PLAIN, 0, DT, TT, MM, k_sweep_plain_std<R, TT, MM>}
 #define LS_PLAIN_GEN(R, DT, TT, MM) {LSAMD_FORM_PLAIN, 1, DT, TT, MM, k_sweep_plain_gen<R, TT, MM>}
 
 const TileEntry kTiles[] = {
-    // split form, both output types
+    // split form (FP64 rotation and sums), both output types
     LS_SPLIT(0, float, LSAMD_F32, 256, 8), LS_SPLIT(0, double, LSAMD_F64, 256, 8),
     LS_SPLIT(0, float, LSAMD_F32, 256, 4), LS_SPLIT(0, double, LSAMD_F64, 256, 4),
+    LS_SPLIT(0, float, LSAMD_F32, 256, 16), LS_SPLIT(0, double, LSAMD_F64, 256, 16),
     LS_SPLIT(0, float, LSAMD_F32, 128, 8), LS_SPLIT(0, double, LSAMD_F64, 128, 8),
     LS_SPLIT(0, float, LSAMD_F32, 512, 8), LS_SPLIT(0, double, LSAMD_F64, 512, 8),
     LS_SPLIT(1, float, LSAMD_F32, 256, 4), LS_SPLIT(1, double, LSAMD_F64, 256, 4),
     LS_SPLIT(1, float, LSAMD_F32, 256, 8), LS_SPLIT(1, double, LSAMD_F64, 256, 8),
-    LS_SPLIT(1, float, LSAMD_F32, 128, 4), LS_SPLIT(1, double, LSAMD_F64, 128, 4),
+    LS_SPLIT(1, float, LSAMD_F32, 128, 8), LS_SPLIT(1, double, LSAMD_F64, 128, 8),
+    // split32 form (FP32 rotation and chunk sums)
+    LS_SPLIT32(0, float, LSAMD_F32, 256, 8), LS_SPLIT32(0, double, LSAMD_F64, 256, 8),
+    LS_SPLIT32(0, float, LSAMD_F32, 256, 4), LS_SPLIT32(0, double, LSAMD_F64, 256, 4),
+    LS_SPLIT32(1, float, LSAMD_F32, 256, 4), LS_SPLIT32(1, double, LSAMD_F64, 256, 4),
     // plain form
     LS_PLAIN_STD(float, LSAMD_F32, 256, 1), LS_PLAIN_STD(double, LSAMD_F64, 256, 1),
     LS_PLAIN_STD(float, LSAMD_F32, 256, 2), LS_PLAIN_STD(double, LSAMD_F64, 256, 2),
@@ -681,11 +824,13 @@ const TileEntry *find_tile(int form, int variant, int dtype, int T, int M) {
 }
 
 // user overrides: [form][variant] -> (T, M); 0 = built-in
-int g_tile_T[2][2] = {{0, 0}, {0, 0}};
-int g_tile_M[2][2] = {{0, 0}, {0, 0}};
+constexpr int kForms = 3;
+int g_tile_T[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
+int g_tile_M[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
 
 Tile default_tile(int form, int variant) {
     if (form == LSAMD_FORM_SPLIT) return variant ? Tile{256, 4} : Tile{256, 8};
+    if (form == LSAMD_FORM_SPLIT32) return variant ? Tile{256, 4} : Tile{256, 8};
     return Tile{256, 1};
 }
 
@@ -700,7 +845,7 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 Tile current_tile(int form, int variant, int dtype) {
     (void)dtype;
-    if (form < 0 || form > 1 || variant < 0 || variant > 1) return Tile{0, 0};
+    if (form < 0 || form >= kForms || variant < 0 || variant > 1) return Tile{0, 0};
     if (g_tile_T[form][variant] > 0) return Tile{g_tile_T[form][variant], g_tile_M[form][variant]};
     return default_tile(form, variant);
 }
@@ -733,6 +878,7 @@ int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
     } else {
         w->rec_t = take(8 * n);
         w->rec_a = take(16 * n);
+        w->rec_r = take(16 * n);
     }
     const size_t parts = (size_t)K * w->chunks * kPeaks;
     w->part_pow = take(8 * parts);
@@ -755,7 +901,7 @@ size_t lsamd_sweep_workspace_size(const lsamd_sweep_args *a) {
 }
 
 int lsamd_set_tile(int form, int variant, int threads, int fpt) {
-    if (form < 0 || form > 1 || variant < 0 || variant > 1)
+    if (form < 0 || form >= kForms || variant < 0 || variant > 1)
         return set_error(LSAMD_ERR_INVALID, "lsamd_set_tile: bad form/variant");
     if (threads == 0 || fpt == 0) {
         g_tile_T[form][variant] = g_tile_M[form][variant] = 0;
@@ -799,7 +945,7 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: bad variant");
     if (a->dtype != LSAMD_F32 && a->dtype != LSAMD_F64)
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: bad dtype");
-    if (a->form != LSAMD_FORM_PLAIN && a->form != LSAMD_FORM_SPLIT)
+    if (a->form != LSAMD_FORM_PLAIN && a->form != LSAMD_FORM_SPLIT && a->form != LSAMD_FORM_SPLIT32)
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: bad form");
     if (a->num_objects == 0 || a->num_freqs == 0 || a->num_obs == 0)
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: empty input (objects, frequencies and "
@@ -830,7 +976,7 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     double *obj_gen = reinterpret_cast<double *>(ws + w.obj_gen);
     const bool f64 = a->dtype == LSAMD_F64;
     const bool gen = a->variant == LSAMD_GENERALIZED;
-    const bool split = a->form == LSAMD_FORM_SPLIT;
+    const bool split = a->form != LSAMD_FORM_PLAIN;
     const bool plain_pre = !split && gen && a->precenter;
     const uint32_t K = a->num_objects;
     const int T = w.tile.threads, M = w.tile.fpt;
@@ -858,6 +1004,7 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     if (split) {
         ra.rec_t = reinterpret_cast<double *>(ws + w.rec_t);
         ra.rec_a = reinterpret_cast<float4 *>(ws + w.rec_a);
+        ra.rec_r = reinterpret_cast<double2 *>(ws + w.rec_r);
         ra.step_rev = (double)T * dw * kInv2Pi;
         void (*fn)(RecArgs) = f64 ? (gen ? k_records_split<double, true> : k_records_split<double, false>)
                                   : (gen ? k_records_split<float, true> : k_records_split<float, false>);
@@ -894,6 +1041,7 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     }
     sw.rec_t = split ? reinterpret_cast<const double *>(ws + w.rec_t) : nullptr;
     sw.rec_a = split ? reinterpret_cast<const float4 *>(ws + w.rec_a) : nullptr;
+    sw.rec_r = split ? reinterpret_cast<const double2 *>(ws + w.rec_r) : nullptr;
     sw.fmin_rev = a->min_freq * kInv2Pi;
     sw.df_rev = dw * kInv2Pi;
     sw.obj_gen = obj_gen;

@@ -23,8 +23,8 @@ Behaviour kept from the reference:
 Differences (all additive): the libraries are loaded from this package's
 ``lib/`` directory (or ``$LSAMD_LIB_DIR``) instead of the working directory;
 a failed library call raises ``RuntimeError`` instead of killing the
-process; an optional ``form="split"`` keyword selects the FP32-split GPU
-arithmetic (default: the reference's arithmetic).
+process; an optional ``form="split"`` (or ``"split32"``) keyword selects
+the fast GPU arithmetic (default: the reference's arithmetic).
 """
 from __future__ import annotations
 
@@ -116,8 +116,10 @@ def _form_env(form):
     if form is None:
         yield
         return
+    if form not in ("plain", "split", "split32"):
+        raise ValueError("form must be 'plain', 'split' or 'split32'")
     old = os.environ.get("LSAMD_FORM")
-    os.environ["LSAMD_FORM"] = "split" if form == "split" else "plain"
+    os.environ["LSAMD_FORM"] = form
     try:
         yield
     finally:

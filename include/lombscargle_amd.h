@@ -51,7 +51,7 @@ extern "C" {
  *                       2/(N sigma^2) like the reference Python builds
  *
  * GPU modes compute in the reference's own arithmetic ("plain" form) unless
- * the environment variable LSAMD_FORM=split selects the FP32-split form.
+ * the environment variable LSAMD_FORM selects "split" or "split32".
  * Errors are reported on stderr and through lsamd_last_error*(); the call
  * returns instead of exiting the process (LSAMD_EXIT_ON_ERROR=1 restores
  * the reference's exit).
@@ -70,10 +70,15 @@ enum lsamd_form {
      * sin/cos, sequential DTYPE sums, tau form; matches the reference CPU
      * path of the same DTYPE */
     LSAMD_FORM_PLAIN = 0,
-    /* FP64 phase reduction, FP32 sin/cos with an FP32 rotation recurrence
-     * across frequencies, FP32 chunk sums folded into FP64, tau-free FP64
-     * power: within 1e-5 of FP64 on the bins that carry power */
-    LSAMD_FORM_SPLIT = 1
+    /* FP64 phase reduction, FP32 hardware sin/cos seeds with an FP64
+     * residual correction, FP64 rotation recurrence across frequencies and
+     * FP64 sums, tau-free FP64 power: within ~3e-6 of FP64 on the bins that
+     * carry power, including aliasing frequencies (default fast form) */
+    LSAMD_FORM_SPLIT = 1,
+    /* as SPLIT but with an FP32 rotation recurrence and FP32 sums over 64
+     * observations folded into FP64: within 1e-5 of FP64 on well-conditioned
+     * bins, up to ~6e-5 at strongly aliased ones */
+    LSAMD_FORM_SPLIT32 = 2
 };
 enum lsamd_status {
     LSAMD_OK = 0,

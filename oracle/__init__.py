@@ -76,6 +76,7 @@ def lib():
         L.lso_object_ranges.argtypes = [POINTER(c_uint), c_uint, POINTER(c_uint), POINTER(c_uint),
                                         POINTER(c_uint)]
         L.lso_object_ranges.restype = c_uint
+        L.lso_set_float_libm_cr.argtypes = [c_int]
         _lib = L
     return _lib
 
@@ -87,6 +88,23 @@ def _sfx(dtype):
     if dtype == np.float32:
         return "f", c_float, np.float32
     raise ValueError(f"unsupported dtype {dtype}")
+
+
+def set_float_libm_cr(on):
+    """Float oracle: glibc sinf/cosf/atan2f (False, the reference's libm) or
+    correctly rounded versions (True)."""
+    lib().lso_set_float_libm_cr(int(bool(on)))
+
+
+class float_libm_cr:
+    """Context manager: run the float oracle with correctly rounded sin/cos/atan2."""
+
+    def __enter__(self):
+        set_float_libm_cr(True)
+
+    def __exit__(self, *exc):
+        set_float_libm_cr(False)
+        return False
 
 
 def object_ranges(obj_id):

@@ -75,12 +75,20 @@ unsigned lso_object_ranges(const unsigned *objId, unsigned n, unsigned *idxMin,
 #undef SQRTR
 
 /* float instantiation: the reference's C++ resolves sin/cos/atan2/sqrt of a
- * float argument to the float overloads */
+ * float argument to the float overloads, i.e. glibc's sinf/cosf/atan2f.
+ * lso_set_float_libm_cr(1) swaps those three for correctly rounded versions
+ * (evaluated in double, rounded once) so tests can measure how much the
+ * reference's float algorithm depends on its libm; default 0 = glibc. */
+static int g_float_cr = 0;
+void lso_set_float_libm_cr(int on) { g_float_cr = on != 0; }
+static float sinf_sel(float x) { return g_float_cr ? (float)sin((double)x) : sinf(x); }
+static float cosf_sel(float x) { return g_float_cr ? (float)cos((double)x) : cosf(x); }
+static float atan2f_sel(float y, float x) { return g_float_cr ? (float)atan2((double)y, (double)x) : atan2f(y, x); }
 #define REAL float
 #define SFX f
-#define SINR sinf
-#define COSR cosf
-#define ATAN2R atan2f
+#define SINR sinf_sel
+#define COSR cosf_sel
+#define ATAN2R atan2f_sel
 #define SQRTR sqrtf
 #include "ls_oracle_body.h"
 #undef REAL

@@ -4,11 +4,18 @@ oracle and the golden fixtures of the reference's own CPU build.
 Tolerances (stated per test):
   * plain form, FP64: relative 1e-12 on every bin whose power is at least
     1e-6 of the object's maximum; |dP| <= 1e-12 * P_max on all bins;
-  * plain form, FP32: relative 1e-5 on bins with P >= 1e-2 * P_max and
-    |dP| <= 1e-5 * P_max on all bins (FP32 sums of ~100-1000 terms with
-    one-ulp sin/cos differences; near-zero bins are ill-conditioned);
+  * plain form, FP32: the GPU evaluates sin/cos/atan2 of float arguments
+    correctly rounded; against the oracle run with correctly rounded float
+    libm: relative 1e-5 on bins with P >= 1e-2 * P_max and |dP| <= 1e-5 P_max.
+    Against the oracle with glibc's sinf/cosf/atan2f (the reference as built):
+    the same bound on >= 99.9% of those bins and relative <= 2e-2 on the
+    rest -- the reference's own FP32 generalized output moves by up to 7.4e-3
+    when only its libm is made correctly rounded (test_oracle), because a
+    1-ulp change of tau flips the float rounding of t - tau at large t;
   * split form vs the FP64 oracle: relative 1e-5 on bins with
     P >= 1e-3 * P_max, |dP| <= 2e-6 * P_max on all bins;
+  * split32 form (FP32 sums): the same on well-conditioned data; relative
+    1e-4 on the aliasing-dominated SDSS data;
   * argmax: identical wherever the best two bins differ by more than the
     tolerance; top-k indices exact against the reference rule applied to
     the GPU's own periodogram.
@@ -69,6 +76,25 @@ def first_strict_max(row):
 TOL = {"double": (1e-12, 1e-6, 1e-12), "float": (1e-5, 1e-2, 1e-5)}
 
 
+def assert_plain(got, dtype, run_oracle):
+    """Plain-form parity.  run_oracle() computes the oracle result with the
+    current float-libm setting."""
+    if dtype in ("double", F64):
+        assert_close(got, run_oracle(), *TOL["double"])
+        return
+    with oracle.float_libm_cr():
+        ref_cr = run_oracle()
+    assert_close(got, ref_cr, *TOL["float"])
+    ref = run_oracle()  # glibc, as the reference is built
+    got2, ref2 = np.atleast_2d(got).astype(F64), np.atleast_2d(ref).astype(F64)
+    pmax = ref2.max(axis=1, keepdims=True)
+    mask = ref2 >= 1e-2 * pmax
+    rel = np.abs(got2 - ref2)[mask] / ref2[mask]
+    frac = float((rel > 1e-5).mean())
+    print(f"vs glibc float libm: max rel {rel.max():.3e}, fraction > 1e-5: {frac:.2e}")
+    assert frac <= 1e-3 and rel.max() <= 2e-2
+
+
 @pytest.mark.parametrize("dtype", ["double", "float"])
 @pytest.mark.parametrize("error", [False, True])
 def test_dropin_gpu_batch_mode1(dtype, error):
@@ -76,11 +102,12 @@ def test_dropin_gpu_batch_mode1(dtype, error):
     ids, per, pg = ls.lombscargle(o, t, y.copy(), 0.1, 10.0, error, "GPU", dy if error else None,
                                   3000, dtype, False)
     npt = F32 if dtype == "float" else F64
-    ref = oracle.batch(o, t, y, dy if error else None, TWO_PI * 0.1, TWO_PI * 10.0, 3000, error,
-                       npt, nthreads=8)
-    assert pg.shape == ref[0].shape and pg.dtype == npt
-    assert_close(pg, ref[0], *TOL[dtype])
-    assert argmax_agrees(pg, ref[0], 10 * TOL[dtype][0]) == 0
+    run = lambda: oracle.batch(o, t, y, dy if error else None, TWO_PI * 0.1, TWO_PI * 10.0, 3000,
+                               error, npt, nthreads=8)[0]
+    ref = run()
+    assert pg.shape == ref.shape and pg.dtype == npt
+    assert_plain(pg, dtype, run)
+    assert argmax_agrees(pg, ref, 10 * TOL[dtype][0]) == 0
 
 
 @pytest.mark.parametrize("dtype", ["double", "float"])
@@ -94,9 +121,8 @@ def test_dropin_gpu_single_mode2(dtype, error, fname):
     ids, per, pg = ls.lombscargle(o, t, y.copy(), 0.1, 10.0, error, "GPU", dy if error else None,
                                   4000, dtype, False)
     npt = F32 if dtype == "float" else F64
-    ref = oracle.single(t, y, dy if error else None, TWO_PI * 0.1, TWO_PI * 10.0, 4000, error, npt,
-                        nthreads=8)
-    assert_close(pg[0], ref[0], *TOL[dtype])
+    assert_plain(pg[0], dtype, lambda: oracle.single(t, y, dy if error else None, TWO_PI * 0.1,
+                                                     TWO_PI * 10.0, 4000, error, npt, nthreads=8)[0])
 
 
 @pytest.mark.parametrize("name,dtype", [("double", F64), ("float", F32)])
@@ -107,7 +133,13 @@ def test_golden_fixtures_on_gpu(golden, name, dtype):
     assert_close(r["pgram"][0], golden[f"s8205_std_{name}"], *TOL[name])
     r = _native.batch_host(None, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 10000, _native.GENERALIZED,
                            dtype, _native.FORM_PLAIN)
-    assert_close(r["pgram"][0], golden[f"s8205_gen_{name}"], *TOL[name])
+    if name == "double":
+        assert_close(r["pgram"][0], golden[f"s8205_gen_{name}"], *TOL[name])
+    else:  # the fixture is the glibc build; see the module docstring
+        assert_plain(r["pgram"][0], dtype, lambda: oracle.single(
+            t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 10000, True, F32)[0])
+        rel = np.abs(r["pgram"][0] - golden["s8205_gen_float"]) / golden["s8205_gen_float"]
+        assert np.median(rel) < 1e-6
     o, t, y, _ = load_data("243_normalized_ztf_filter2.txt")
     r = _native.batch_host(None, t, y, None, 3.142, 150.796, 10000, _native.STANDARD, dtype,
                            _native.FORM_PLAIN)
@@ -121,11 +153,18 @@ def test_golden_fixtures_on_gpu(golden, name, dtype):
         key = f"sdss_{'gen' if err else 'std'}_{name}"
         pg = r["pgram"]
         got_win = np.take_along_axis(pg, golden[key + "_win_index"], 1)
-        assert_close(got_win, golden[key + "_win_values"], *TOL[name])
         got_fix = pg[:, golden["sdss_fixed_index"]]
-        rr, aa = rel_report(got_fix, golden[key + "_fixed_values"], TOL[name][1])
-        assert rr <= TOL[name][0]
-        assert np.array_equal(r["index"][:, 0], golden[key + "_argmax"])
+        if name == "double" or not err:
+            assert_close(got_win, golden[key + "_win_values"], *TOL[name])
+            rr, aa = rel_report(got_fix, golden[key + "_fixed_values"], TOL[name][1])
+            assert rr <= TOL[name][0]
+        else:  # FP32 generalized: libm-sensitive at MJD-scale t (module docstring)
+            for got_v, ref_v in ((got_win, golden[key + "_win_values"]),
+                                 (got_fix, golden[key + "_fixed_values"])):
+                rel = np.abs(got_v - ref_v) / np.abs(ref_v)
+                assert np.median(rel) < 1e-6 and np.nanmax(rel) < 2e-2
+        same = np.mean(r["index"][:, 0] == golden[key + "_argmax"])
+        assert same >= (1.0 if name == "double" else 0.97)
 
 
 # --------------------------------------------------------------------------
@@ -150,13 +189,38 @@ def test_split_form_vs_fp64_oracle(golden, error, dtype):
     assert argmax_agrees(r["pgram"], ref[0], 1e-5) == 0
 
 
-def test_split_form_reference_data_sdss_and_ztf():
+@pytest.mark.parametrize("error", [False, True])
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_split32_form_vs_fp64_oracle(golden, error, dtype):
+    tag = "gen" if error else "std"
+    o, t, y = golden[f"syn_{tag}_obj"], golden[f"syn_{tag}_t"], golden[f"syn_{tag}_y"]
+    dy = golden[f"syn_{tag}_dy"] if error else None
+    t, y = t.astype(dtype).astype(F64), y.astype(dtype).astype(F64)
+    dy = None if dy is None else dy.astype(dtype).astype(F64)
+    ref = oracle.batch(o, t, y, dy, synth.OMEGA_MIN, synth.OMEGA_MAX, 20000, error, F64, nthreads=8)
+    r = _native.batch_host(o, t, y, dy, synth.OMEGA_MIN, synth.OMEGA_MAX, 20000,
+                           _native.GENERALIZED if error else _native.STANDARD, dtype,
+                           _native.FORM_SPLIT32)
+    assert_close(r["pgram"], ref[0], 1e-5, 1e-3, 2e-6)
+    assert argmax_agrees(r["pgram"], ref[0], 1e-5) == 0
+
+
+def test_split_forms_on_aliased_sdss_data():
+    """SDSS stripe 82 sampling makes the 2x2 normal matrix near-singular at
+    the daily aliases (e.g. ~2.003 cycles/day): FP32 sums lose ~1e-4 there,
+    FP64 rotation and sums do not."""
     o, t, y, dy = load_data("SDSS_stripe82_band_z.txt")
     ref = oracle.batch(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 5000, True, F64, nthreads=8)
     r = _native.batch_host(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 5000, _native.GENERALIZED,
                            F64, _native.FORM_SPLIT)
     assert_close(r["pgram"], ref[0], 1e-5, 1e-3, 2e-6)
-    o, t, y, _ = load_data("243_normalized_ztf_filter2.txt")  # JD-scale times
+    r = _native.batch_host(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 5000, _native.GENERALIZED,
+                           F64, _native.FORM_SPLIT32)
+    assert_close(r["pgram"], ref[0], 1e-4, 1e-3, 2e-5)
+
+
+def test_split_form_jd_scale_times():
+    o, t, y, _ = load_data("243_normalized_ztf_filter2.txt")  # t ~ 2.4587e6 (JD)
     ref = oracle.single(t, y, None, 3.142, 150.796, 20000, False, F64, nthreads=8)
     r = _native.batch_host(None, t, y, None, 3.142, 150.796, 20000, _native.STANDARD, F64,
                            _native.FORM_SPLIT)
@@ -166,7 +230,7 @@ def test_split_form_reference_data_sdss_and_ztf():
 # --------------------------------------------------------------------------
 # fused peaks (argmax / top-3) against the reference rule
 # --------------------------------------------------------------------------
-@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT])
+@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT, _native.FORM_SPLIT32])
 def test_peaks_follow_reference_rule(form):
     b = synth.lognormal_batch(40, 140, 0.6, 20, 1000, 1800, False, 21)
     nf = 7777
@@ -201,7 +265,7 @@ def test_peaks_ties_and_nonpositive():
 # --------------------------------------------------------------------------
 # edge cases: staging boundaries, ragged grids, tiny and degenerate objects
 # --------------------------------------------------------------------------
-@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT])
+@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT, _native.FORM_SPLIT32])
 @pytest.mark.parametrize("nf", [1, 3, 255, 2049, 4097])
 def test_ragged_grids_and_staging_boundaries(form, nf):
     rng = np.random.default_rng(nf)
@@ -247,15 +311,19 @@ def test_memory_budget_batching_is_exact():
     assert np.array_equal(full["index"], part["index"])
 
 
-def test_identical_times_degenerate_like_reference():
+@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT, _native.FORM_SPLIT32])
+def test_identical_times_zero_sum_gives_zero_power(form):
+    """All times equal and sum(y) = 0: every basis vector is constant, so the
+    exact power is 0; every form returns rounding noise around 0 (the tau-free
+    forms may return NaN, 0/0, where the matrix is exactly singular)."""
     t = np.full(10, 3.5)
     y = np.linspace(-1, 1, 10)
-    ref = oracle.single(t, y, None, 1.0, 5.0, 50, False, F64, normalize=False)
-    r = _native.batch_host(None, t, y, None, 1.0, 5.0, 50, _native.STANDARD, F64, _native.FORM_PLAIN,
+    ref = oracle.single(t, y, None, 1.0, 5.0, 50, False, F64, normalize=False)[0]
+    assert np.all(np.abs(ref) < 1e-12)
+    r = _native.batch_host(None, t, y, None, 1.0, 5.0, 50, _native.STANDARD, F64, form,
                            normalize=False)
-    same = (np.isnan(ref[0]) & np.isnan(r["pgram"][0])) | (ref[0] == r["pgram"][0]) | \
-           (np.abs(ref[0] - r["pgram"][0]) <= 1e-12 * np.abs(ref[0]))
-    assert same.mean() > 0.9
+    g = r["pgram"][0]
+    assert np.all(np.isnan(g) | (np.abs(g) < 1e-9))
 
 
 # --------------------------------------------------------------------------

@@ -337,3 +337,24 @@ def test_degenerate_two_points_not_finite():
     assert np.isfinite(p) or np.isnan(p) or np.isinf(p)
     if oracle.ref_available(False, F64):
         assert bits_equal(np.array(p), np.array(oracle.ref_power(1.3, t, y, None, False)))
+
+
+def test_reference_float_generalized_is_libm_sensitive():
+    """The reference's FP32 generalized algorithm depends on its libm: at
+    MJD-scale times a 1-ulp change of tau flips the float rounding of t - tau
+    (ulp 0.004 d at 5e4 d).  Swapping glibc's sinf/cosf/atan2f for correctly
+    rounded ones moves a few bins by ~1e-2 while leaving almost all within
+    1e-5; FP64 is unaffected.  This bounds what any other float libm (the
+    GPU's included) can be expected to reproduce."""
+    o, t, y, dy = load_data("SDSS_stripe82_band_z.txt")
+    a = oracle.batch(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 3000, True, F32, nthreads=8)[0]
+    with oracle.float_libm_cr():
+        b = oracle.batch(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 3000, True, F32, nthreads=8)[0]
+        d64 = oracle.batch(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 300, True, F64, nthreads=8)[0]
+    a64 = oracle.batch(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 300, True, F64, nthreads=8)[0]
+    a, b = a.astype(F64), b.astype(F64)
+    mask = a >= 1e-2 * a.max(axis=1, keepdims=True)
+    rel = np.abs(a - b)[mask] / a[mask]
+    assert rel.max() > 1e-3            # some bins move a lot ...
+    assert (rel > 1e-5).mean() < 1e-4  # ... but very few
+    assert np.array_equal(a64, d64)    # the switch only touches the float build
