@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nf", type=int, default=1000000)
+    ap.add_argument("--profile-once", action="store_true",
+                    help="for rocprofv3 counter runs: one 1 GiB device copy (calibration of the "
+                         "HBM byte counters), then one sweep of each split form; no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -119,6 +122,19 @@ def main():
         t1 = time.perf_counter()
         L.lsamd_set_timing(0)
         return (t1 - t0) * 1e3 / steps, float(np.mean(kms))
+
+    if args.profile_once:
+        src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)  # 1 GiB
+        dst = src.clone()
+        torch.cuda.synchronize()
+        for form in (_native.FORM_SPLIT32, _native.FORM_SPLIT):
+            plan = make(form)
+            plan.run()
+            torch.cuda.synchronize()
+            del plan
+        log("profile-once: 1 GiB copy, one split32 sweep, one split sweep")
+        del src, dst
+        return
 
     split = make(_native.FORM_SPLIT)
     T, M = ctypes.c_int(), ctypes.c_int()
