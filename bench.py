@@ -70,6 +70,135 @@ def cpu_baseline(data, nf, sample_objects, threads):
             "objects_per_s": sample_objects / dt}
 
 
+def full_accuracy(torch, res, ref, K):
+    """Split-form output vs the FP64 form over every bin."""
+    a, b = res["pgram"], ref["pgram"]
+    pmax = b.max(dim=1, keepdim=True).values
+    mask = b >= 1e-3 * pmax
+    rel = ((a - b).abs() / b.abs()).where(mask, torch.zeros_like(a))
+    out = {"max_rel_err_masked": float(rel.max()),
+           "median_rel_err": float(((a - b).abs() / b.abs()).flatten()[::97].median()),
+           "max_abs_err_over_pmax": float(((a - b).abs() / pmax).max()),
+           "argmax_identical": f"{int((res['index'] == ref['index']).sum())}/{K}",
+           "sum_periods": float(res["period"].sum()),
+           "sum_periods_fp64": float(ref["period"].sum()),
+           "mask": "P >= 1e-3 * P_max per object"}
+    del rel, mask
+    return out
+
+
+def timed_plan(torch, L, plan, steps, warmup, stream=None):
+    for _ in range(warmup):
+        plan.run(stream)
+    torch.cuda.synchronize()
+    L.lsamd_set_timing(1)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.run(stream)
+        torch.cuda.synchronize()
+        kms.append(L.lsamd_last_sweep_kernel_ms())
+    t1 = time.perf_counter()
+    L.lsamd_set_timing(0)
+    return (t1 - t0) * 1e3 / steps, float(np.mean(kms))
+
+
+def generalized_batch(torch, L, dev, steps):
+    """1000 objects over 2000 d, un-centred magnitudes (15..20) with dmag:
+    split forms against the FP64 generalized form, 1e6 bins."""
+    from gpu_lomb_scargle_amd import _native, device, synth
+    data = synth.generalized_batch_1000()
+    K = int(data["counts"].size)
+    nf = 1000000
+    terms = float(data["counts"].sum()) * nf
+    t, y, dy, off = device.numpy_to_device(data["obj_id"], data["t"], data["y"], data["dy"],
+                                           np.float64, dev)
+    mk = lambda form: device.SweepPlan(t, y, dy, off, min_freq=synth.OMEGA_MIN,
+                                       max_freq=synth.OMEGA_MAX, num_freqs=nf,
+                                       variant=_native.GENERALIZED, form=form, top_k=1)
+    out = {"workload": "generalized L-S batch: 1000 objects (median 140 obs), 2000 d span, "
+                       "magnitudes 15..20 not subtracted, dmag logU(0.01, 0.2), 1e6 bins",
+           "terms_per_step": terms}
+    plain = mk(_native.FORM_PLAIN)
+    ms, kms = timed_plan(torch, L, plain, 1, 1)
+    out["fp64_plain"] = {"value": terms / (ms / 1e3), "ms_per_step": round(ms, 3)}
+    ref = plain.results()
+    for name, form in (("split", _native.FORM_SPLIT), ("split32", _native.FORM_SPLIT32)):
+        p = mk(form)
+        ms, kms = timed_plan(torch, L, p, steps, 1)
+        out[name] = {"value": terms / (ms / 1e3), "ms_per_step": round(ms, 3),
+                     "kernel_ms": round(kms, 3), "accuracy": full_accuracy(torch, p.results(), ref, K)}
+        del p
+    del plain, ref
+    torch.cuda.empty_cache()
+    return out
+
+
+def uneven_batch(torch, L, dev, steps):
+    """100000 objects, N log-normal (median 60, sigma 1, 5..5000), 2e5 bins,
+    top-3 peaks, no periodogram: persistent + graph-captured + autotuned vs
+    one workgroup per item, vs a per-object launch loop."""
+    from ctypes import byref, c_void_p
+    from gpu_lomb_scargle_amd import _native, device, synth
+    out = {"workload": "uneven batch: 100000 objects, log-normal N (median 60, sigma 1.0, 5..5000), "
+                       "2e5 angular freqs 1.005..150.796, top-3 peaks per object, no periodogram"}
+    nf = 200000
+    for vname, variant in (("standard", _native.STANDARD), ("generalized", _native.GENERALIZED)):
+        data = synth.uneven_batch_100000(with_error=variant == _native.GENERALIZED)
+        K = int(data["counts"].size)
+        rows = int(data["counts"].sum())
+        terms = float(rows) * nf
+        t, y, dy, off = device.numpy_to_device(data["obj_id"], data["t"], data["y"], data["dy"],
+                                               np.float64, dev)
+        res = {"objects": K, "rows": rows, "terms_per_step": terms}
+        form = _native.FORM_SPLIT32
+        kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=nf, variant=variant,
+                  form=form, top_k=3, pgram=False)
+        pers = device.SweepPlan(t, y, dy, off, schedule=_native.SCHEDULE_PERSISTENT, **kw)
+        T, M = pers.autotune()
+        stream = torch.cuda.Stream()
+        pers.capture_graph(stream)
+        ms, kms = timed_plan(torch, L, pers, steps, 1, stream)
+        res["persistent_graph"] = {"value": terms / (ms / 1e3), "objects_per_s": K / (ms / 1e3),
+                                   "ms_per_step": round(ms, 3), "tile": f"{T}x{M}"}
+        grid = device.SweepPlan(t, y, dy, off, schedule=_native.SCHEDULE_GRID, **kw)
+        ms_g, _ = timed_plan(torch, L, grid, steps, 1)
+        res["grid"] = {"value": terms / (ms_g / 1e3), "ms_per_step": round(ms_g, 3)}
+        same = bool(torch.equal(pers.results()["index"], grid.results()["index"]))
+        # the per-object launch loop: one lsamd_sweep (4 kernels) per object
+        a = _native.SweepArgs()
+        ctypes_mod = __import__("ctypes")
+        ctypes_mod.memmove(byref(a), byref(pers.args), ctypes_mod.sizeof(a))
+        a.num_objects, a.order, a.schedule = 1, None, _native.SCHEDULE_GRID
+        ws1 = torch.empty(int(L.lsamd_sweep_workspace_size(byref(a))), dtype=torch.uint8, device=dev)
+        idx = torch.zeros(K * 3, dtype=torch.int32, device=dev)
+        pw = torch.zeros(K * 3, dtype=torch.float64, device=dev)
+        pe = torch.zeros(K * 3, dtype=torch.float64, device=dev)
+        s0 = torch.cuda.current_stream().cuda_stream
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        base_off, base_i, base_p, base_e = off.data_ptr(), idx.data_ptr(), pw.data_ptr(), pe.data_ptr()
+        for k in range(K):
+            a.offsets = base_off + 8 * k
+            a.peak_index = base_i + 12 * k
+            a.peak_power = base_p + 24 * k
+            a.peak_period = base_e + 24 * k
+            rc = L.lsamd_sweep(byref(a), c_void_p(ws1.data_ptr()), ws1.numel(), c_void_p(s0))
+            if rc:
+                raise RuntimeError(L.lsamd_last_error())
+        torch.cuda.synchronize()
+        ms_loop = (time.perf_counter() - t0) * 1e3
+        loop_same = bool(torch.equal(idx.view(K, 3), grid.results()["index"]))
+        res["per_object_loop"] = {"value": terms / (ms_loop / 1e3), "ms": round(ms_loop, 1),
+                                  "launches": K}
+        res["persistent_speedup_vs_loop"] = round(ms_loop / ms, 1)
+        res["peaks_identical_persistent_grid_loop"] = same and loop_same
+        out[vname] = res
+        del pers, grid, ws1
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +209,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nf", type=int, default=1000000)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the generalized and uneven-batch workloads")
     ap.add_argument("--profile-once", action="store_true",
                     help="for rocprofv3 counter runs: one 1 GiB device copy (calibration of the "
                          "HBM byte counters), then one sweep of each split form; no JSON")
@@ -208,6 +339,13 @@ def main():
                     "form": "FP64 phase, FP32 sincos, FP32 rotation and 64-obs sums folded into FP64"},
         "roofline": roofline,
     }
+    if not args.no_extra:
+        del split, split32, plain, pp, b, mask
+        torch.cuda.empty_cache()
+        out["generalized_batch"] = generalized_batch(torch, L, dev, max(2, args.steps // 2))
+        log(f"generalized batch: {out['generalized_batch']}")
+        out["uneven_batch"] = uneven_batch(torch, L, dev, max(2, args.steps // 2))
+        log(f"uneven batch: {out['uneven_batch']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(data, nf, args.cpu_objects, args.cpu_threads)
         log(f"cpu baseline: {out['cpu_baseline']}")

@@ -27,6 +27,7 @@ STANDARD, GENERALIZED = 0, 1
 F32, F64 = 0, 1
 FORM_PLAIN, FORM_SPLIT, FORM_SPLIT32 = 0, 1, 2
 OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_NODEVICE = 0, 1, 2, 3, 4
+SCHEDULE_GRID, SCHEDULE_PERSISTENT = 0, 1
 
 # Every function include/lombscargle_amd.h declares (LombScarglePy only in the
 # drop-in libraries).
@@ -35,6 +36,8 @@ EXPORTED = [
     "lsamd_cpu_single", "lsamd_count_objects", "lsamd_object_offsets", "lsamd_last_error_code",
     "lsamd_last_error", "lsamd_clear_error", "lsamd_device_count", "lsamd_set_timing",
     "lsamd_last_sweep_kernel_ms", "lsamd_set_tile", "lsamd_get_tile", "lsamd_version",
+    "lsamd_graph_create", "lsamd_graph_launch", "lsamd_graph_destroy", "lsamd_autotune",
+    "lsamd_autotune_workspace_size",
 ]
 
 
@@ -56,7 +59,7 @@ class SweepArgs(Structure):
         ("top_k", c_uint32), ("num_obs", c_uint64), ("min_freq", c_double), ("max_freq", c_double),
         ("offsets", c_void_p), ("order", c_void_p), ("t", c_void_p), ("y", c_void_p),
         ("dy", c_void_p), ("pgram", c_void_p), ("peak_index", c_void_p), ("peak_power", c_void_p),
-        ("peak_period", c_void_p),
+        ("peak_period", c_void_p), ("schedule", c_int32),
     ]
 
 
@@ -88,6 +91,17 @@ def _declare(L):
     L.lsamd_set_tile.restype = c_int
     L.lsamd_get_tile.argtypes = [c_int, c_int, c_int, POINTER(c_int), POINTER(c_int)]
     L.lsamd_version.restype = c_int
+    L.lsamd_graph_create.argtypes = [POINTER(SweepArgs), c_void_p, c_size_t, c_void_p, POINTER(c_void_p)]
+    L.lsamd_graph_create.restype = c_int
+    L.lsamd_graph_launch.argtypes = [c_void_p, c_void_p]
+    L.lsamd_graph_launch.restype = c_int
+    L.lsamd_graph_destroy.argtypes = [c_void_p]
+    L.lsamd_graph_destroy.restype = None
+    L.lsamd_autotune.argtypes = [POINTER(SweepArgs), c_void_p, c_size_t, c_void_p, POINTER(c_int),
+                                 POINTER(c_int)]
+    L.lsamd_autotune.restype = c_int
+    L.lsamd_autotune_workspace_size.argtypes = [POINTER(SweepArgs)]
+    L.lsamd_autotune_workspace_size.restype = c_size_t
     return L
 
 

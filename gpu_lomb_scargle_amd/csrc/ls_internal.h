@@ -42,6 +42,7 @@ struct Workspace {
     // per (object, chunk) peak partials
     size_t part_pow = 0;    // double[K*chunks*topk]
     size_t part_idx = 0;    // uint32[K*chunks*topk]
+    size_t queue = 0;       // uint64   persistent-schedule item counter
     uint32_t chunks = 0;
     Tile tile{};
 };

@@ -165,7 +165,9 @@ struct SweepArgs {
     const uint64_t *offsets;
     const uint32_t *order;
     uint32_t K, nf, chunks;
-    uint64_t item_base;
+    uint64_t item_base;        // first item of this launch (one workgroup per item)
+    uint64_t total_items;      // K * chunks
+    unsigned long long *queue; // persistent schedule: next item counter, else null
     // plain
     const void *t, *y, *w;
     double fmin_r, fstep_r;    // DTYPE values stored in double
@@ -308,12 +310,31 @@ __global__ void k_records_split(RecArgs a) {
     }
 }
 
-__device__ __forceinline__ void item_coords(const SweepArgs &a, uint32_t &obj, uint32_t &chunk) {
-    const uint64_t item = a.item_base + blockIdx.x;
+__device__ __forceinline__ void item_coords(const SweepArgs &a, uint64_t item, uint32_t &obj,
+                                            uint32_t &chunk) {
     const uint32_t pos = (uint32_t)(item / a.chunks);
     chunk = (uint32_t)(item - (uint64_t)pos * a.chunks);
     obj = a.order ? a.order[pos] : pos;
 }
+
+// Work items: either one per workgroup (item_base + blockIdx.x), or, with a
+// queue, persistent workgroups that take the next item from a counter until
+// the items run out.  Items are numbered in processing order (longest object
+// first), chunk-minor.
+__device__ __forceinline__ uint64_t fetch_item(const SweepArgs &a, uint64_t *s_item) {
+    __syncthreads();  // every thread is done with the previous item and its s_item read
+    if (threadIdx.x == 0) *s_item = atomicAdd(a.queue, 1ull);
+    __syncthreads();
+    return *s_item;
+}
+__device__ __forceinline__ uint64_t first_item(const SweepArgs &a, uint64_t *s_item) {
+    return a.queue ? fetch_item(a, s_item) : a.item_base + blockIdx.x;
+}
+__device__ __forceinline__ uint64_t next_item(const SweepArgs &a, uint64_t *s_item) {
+    return a.queue ? fetch_item(a, s_item) : ~0ull;
+}
+
+__global__ void k_zero_counter(unsigned long long *c) { *c = 0ull; }
 
 template <typename OUT, int T>
 __device__ __forceinline__ void write_partial(const SweepArgs &a, Peaks &L, uint32_t obj,
@@ -340,67 +361,71 @@ __global__ __launch_bounds__(T) void k_sweep_plain_std(SweepArgs a) {
     __shared__ R s_t[T], s_y[T];
     __shared__ double s_pp[T / 64][kPeaks];
     __shared__ uint32_t s_pi[T / 64][kPeaks];
-    uint32_t obj, chunk;
-    item_coords(a, obj, chunk);
-    const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
-    const R *gt = static_cast<const R *>(a.t);
-    const R *gy = static_cast<const R *>(a.y);
-    const uint32_t tid = threadIdx.x;
-    const uint32_t kbase = chunk * (uint32_t)(T * M);
-    const R fmin = (R)a.fmin_r, fstep = (R)a.fstep_r;
-    R om[M], xc[M], xs[M], cc[M], ss[M], cs[M];
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        om[m] = fmin + (fstep * (R)(kbase + m * T + tid));  // main.cu:1611
-        xc[m] = xs[m] = cc[m] = ss[m] = cs[m] = (R)0;
-    }
-    for (uint64_t b0 = beg; b0 < end; b0 += T) {
-        const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
-        __syncthreads();
-        if (tid < cnt) {
-            s_t[tid] = gt[b0 + tid];
-            s_y[tid] = gy[b0 + tid];
+    __shared__ uint64_t s_item;
+    for (uint64_t item = first_item(a, &s_item); item < a.total_items;
+         item = next_item(a, &s_item)) {
+        uint32_t obj, chunk;
+        item_coords(a, item, obj, chunk);
+        const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
+        const R *gt = static_cast<const R *>(a.t);
+        const R *gy = static_cast<const R *>(a.y);
+        const uint32_t tid = threadIdx.x;
+        const uint32_t kbase = chunk * (uint32_t)(T * M);
+        const R fmin = (R)a.fmin_r, fstep = (R)a.fstep_r;
+        R om[M], xc[M], xs[M], cc[M], ss[M], cs[M];
+    #pragma unroll
+        for (int m = 0; m < M; ++m) {
+            om[m] = fmin + (fstep * (R)(kbase + m * T + tid));  // main.cu:1611
+            xc[m] = xs[m] = cc[m] = ss[m] = cs[m] = (R)0;
         }
-        __syncthreads();
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const R tj = s_t[j], yj = s_y[j];
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-                R s, c;
-                sincos_r(om[m] * tj, &s, &c);
-                xc[m] = xc[m] + yj * c;
-                xs[m] = xs[m] + yj * s;
-                cc[m] = cc[m] + c * c;
-                ss[m] = ss[m] + s * s;
-                cs[m] = cs[m] + c * s;
+        for (uint64_t b0 = beg; b0 < end; b0 += T) {
+            const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+            __syncthreads();
+            if (tid < cnt) {
+                s_t[tid] = gt[b0 + tid];
+                s_y[tid] = gy[b0 + tid];
+            }
+            __syncthreads();
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const R tj = s_t[j], yj = s_y[j];
+    #pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    R s, c;
+                    sincos_r(om[m] * tj, &s, &c);
+                    xc[m] = xc[m] + yj * c;
+                    xs[m] = xs[m] + yj * s;
+                    cc[m] = cc[m] + c * c;
+                    ss[m] = ss[m] + s * s;
+                    cs[m] = cs[m] + c * s;
+                }
             }
         }
-    }
-    const double scale = a.obj_scale[obj];
-    R *pg = static_cast<R *>(a.pgram);
-    Peaks L;
-    peaks_init(L);
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        const uint32_t kf = kbase + m * T + tid;
-        const R omega = om[m];
-        const R tau = (R)(atan2(2.0 * (double)cs[m], (double)(cc[m] - ss[m])) / (2.0 * (double)omega));
-        R s_tau, c_tau;
-        sincos_r(omega * tau, &s_tau, &c_tau);
-        const R c_tau2 = c_tau * c_tau, s_tau2 = s_tau * s_tau;
-        const R cs_tau = (R)(2.0 * (double)c_tau * (double)s_tau);
-        const R p1 = c_tau * xc[m] + s_tau * xs[m];
-        const R p2 = c_tau * xs[m] - s_tau * xc[m];
-        const R q = (p1 * p1) / (c_tau2 * cc[m] + cs_tau * cs[m] + s_tau2 * ss[m]) +
-                    (p2 * p2) / (c_tau2 * ss[m] - cs_tau * cs[m] + s_tau2 * cc[m]);
-        R pw = (R)(0.5 * (double)q);
-        pw = (R)((double)pw * scale);  // main.cu:1335 / 1350
-        if (kf < a.nf) {
-            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-            peaks_insert(L, (double)pw, kf);
+        const double scale = a.obj_scale[obj];
+        R *pg = static_cast<R *>(a.pgram);
+        Peaks L;
+        peaks_init(L);
+    #pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const uint32_t kf = kbase + m * T + tid;
+            const R omega = om[m];
+            const R tau = (R)(atan2(2.0 * (double)cs[m], (double)(cc[m] - ss[m])) / (2.0 * (double)omega));
+            R s_tau, c_tau;
+            sincos_r(omega * tau, &s_tau, &c_tau);
+            const R c_tau2 = c_tau * c_tau, s_tau2 = s_tau * s_tau;
+            const R cs_tau = (R)(2.0 * (double)c_tau * (double)s_tau);
+            const R p1 = c_tau * xc[m] + s_tau * xs[m];
+            const R p2 = c_tau * xs[m] - s_tau * xc[m];
+            const R q = (p1 * p1) / (c_tau2 * cc[m] + cs_tau * cs[m] + s_tau2 * ss[m]) +
+                        (p2 * p2) / (c_tau2 * ss[m] - cs_tau * cs[m] + s_tau2 * cc[m]);
+            R pw = (R)(0.5 * (double)q);
+            pw = (R)((double)pw * scale);  // main.cu:1335 / 1350
+            if (kf < a.nf) {
+                if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+                peaks_insert(L, (double)pw, kf);
+            }
         }
+        if (a.want_peaks) write_partial<R, T>(a, L, obj, chunk, s_pp, s_pi);
     }
-    if (a.want_peaks) write_partial<R, T>(a, L, obj, chunk, s_pp, s_pi);
 }
 
 // ---------------------------------------------------------------------------
@@ -413,105 +438,109 @@ __global__ __launch_bounds__(T) void k_sweep_plain_gen(SweepArgs a) {
     __shared__ R s_t[T], s_y[T], s_w[T];
     __shared__ double s_pp[T / 64][kPeaks];
     __shared__ uint32_t s_pi[T / 64][kPeaks];
-    uint32_t obj, chunk;
-    item_coords(a, obj, chunk);
-    const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
-    const R *gt = static_cast<const R *>(a.t);
-    const R *gy = static_cast<const R *>(a.y);
-    const R *gw = static_cast<const R *>(a.w);
-    const uint32_t tid = threadIdx.x;
-    const uint32_t kbase = chunk * (uint32_t)(T * M);
-    const R fmin = (R)a.fmin_r, fstep = (R)a.fstep_r;
-    R om[M], wsum[M], S[M], C[M], S2[M], C2[M], tau[M];
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        om[m] = fmin + (fstep * (R)(kbase + m * T + tid));
-        wsum[m] = S[m] = C[m] = S2[m] = C2[m] = (R)0;
-    }
-    // pass 1: tau
-    for (uint64_t b0 = beg; b0 < end; b0 += T) {
-        const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
-        __syncthreads();
-        if (tid < cnt) {
-            s_t[tid] = gt[b0 + tid];
-            s_w[tid] = gw[b0 + tid];
+    __shared__ uint64_t s_item;
+    for (uint64_t item = first_item(a, &s_item); item < a.total_items;
+         item = next_item(a, &s_item)) {
+        uint32_t obj, chunk;
+        item_coords(a, item, obj, chunk);
+        const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
+        const R *gt = static_cast<const R *>(a.t);
+        const R *gy = static_cast<const R *>(a.y);
+        const R *gw = static_cast<const R *>(a.w);
+        const uint32_t tid = threadIdx.x;
+        const uint32_t kbase = chunk * (uint32_t)(T * M);
+        const R fmin = (R)a.fmin_r, fstep = (R)a.fstep_r;
+        R om[M], wsum[M], S[M], C[M], S2[M], C2[M], tau[M];
+    #pragma unroll
+        for (int m = 0; m < M; ++m) {
+            om[m] = fmin + (fstep * (R)(kbase + m * T + tid));
+            wsum[m] = S[m] = C[m] = S2[m] = C2[m] = (R)0;
         }
-        __syncthreads();
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const R tj = s_t[j], w = s_w[j];
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-                R s, c;
-                wsum[m] += w;
-                sincos_r(om[m] * tj, &s, &c);
-                S[m] += w * s;
-                C[m] += w * c;
-                S2[m] = (R)((double)S2[m] + 2.0 * (double)w * (double)s * (double)c);
-                C2[m] = (R)((double)C2[m] + ((double)w - 2.0 * (double)w * (double)s * (double)s));
+        // pass 1: tau
+        for (uint64_t b0 = beg; b0 < end; b0 += T) {
+            const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+            __syncthreads();
+            if (tid < cnt) {
+                s_t[tid] = gt[b0 + tid];
+                s_w[tid] = gw[b0 + tid];
+            }
+            __syncthreads();
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const R tj = s_t[j], w = s_w[j];
+    #pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    R s, c;
+                    wsum[m] += w;
+                    sincos_r(om[m] * tj, &s, &c);
+                    S[m] += w * s;
+                    C[m] += w * c;
+                    S2[m] = (R)((double)S2[m] + 2.0 * (double)w * (double)s * (double)c);
+                    C2[m] = (R)((double)C2[m] + ((double)w - 2.0 * (double)w * (double)s * (double)s));
+                }
             }
         }
-    }
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        S2[m] /= wsum[m];
-        C2[m] /= wsum[m];
-        S[m] /= wsum[m];
-        C[m] /= wsum[m];
-        S2[m] = (R)((double)S2[m] - 2.0 * (double)S[m] * (double)C[m]);
-        C2[m] -= (C[m] * C[m] - S[m] * S[m]);
-        tau[m] = (R)(0.5 * (double)atan2_r(S2[m], C2[m]) / (double)om[m]);
-    }
-    // pass 2: the tau-shifted sums
-    R Y[M], YY[M], Ct[M], St[M], YC[M], YS[M], CC[M], SS[M];
-#pragma unroll
-    for (int m = 0; m < M; ++m) Y[m] = YY[m] = Ct[m] = St[m] = YC[m] = YS[m] = CC[m] = SS[m] = (R)0;
-    for (uint64_t b0 = beg; b0 < end; b0 += T) {
-        const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
-        __syncthreads();
-        if (tid < cnt) {
-            s_t[tid] = gt[b0 + tid];
-            s_y[tid] = gy[b0 + tid];
-            s_w[tid] = gw[b0 + tid];
+    #pragma unroll
+        for (int m = 0; m < M; ++m) {
+            S2[m] /= wsum[m];
+            C2[m] /= wsum[m];
+            S[m] /= wsum[m];
+            C[m] /= wsum[m];
+            S2[m] = (R)((double)S2[m] - 2.0 * (double)S[m] * (double)C[m]);
+            C2[m] -= (C[m] * C[m] - S[m] * S[m]);
+            tau[m] = (R)(0.5 * (double)atan2_r(S2[m], C2[m]) / (double)om[m]);
         }
-        __syncthreads();
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const R tj = s_t[j], yj = s_y[j], w = s_w[j];
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-                R s, c;
-                sincos_r(om[m] * (tj - tau[m]), &s, &c);
-                Y[m] += w * yj;
-                YY[m] += w * yj * yj;
-                Ct[m] += w * c;
-                St[m] += w * s;
-                YC[m] += w * yj * c;
-                YS[m] += w * yj * s;
-                CC[m] += w * c * c;
-                SS[m] += w * s * s;
+        // pass 2: the tau-shifted sums
+        R Y[M], YY[M], Ct[M], St[M], YC[M], YS[M], CC[M], SS[M];
+    #pragma unroll
+        for (int m = 0; m < M; ++m) Y[m] = YY[m] = Ct[m] = St[m] = YC[m] = YS[m] = CC[m] = SS[m] = (R)0;
+        for (uint64_t b0 = beg; b0 < end; b0 += T) {
+            const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+            __syncthreads();
+            if (tid < cnt) {
+                s_t[tid] = gt[b0 + tid];
+                s_y[tid] = gy[b0 + tid];
+                s_w[tid] = gw[b0 + tid];
+            }
+            __syncthreads();
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const R tj = s_t[j], yj = s_y[j], w = s_w[j];
+    #pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    R s, c;
+                    sincos_r(om[m] * (tj - tau[m]), &s, &c);
+                    Y[m] += w * yj;
+                    YY[m] += w * yj * yj;
+                    Ct[m] += w * c;
+                    St[m] += w * s;
+                    YC[m] += w * yj * c;
+                    YS[m] += w * yj * s;
+                    CC[m] += w * c * c;
+                    SS[m] += w * s * s;
+                }
             }
         }
-    }
-    R *pg = static_cast<R *>(a.pgram);
-    Peaks L;
-    peaks_init(L);
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        const uint32_t kf = kbase + m * T + tid;
-        const R ws = wsum[m];
-        R y_ = Y[m] / ws, yy = YY[m] / ws, ct = Ct[m] / ws, st = St[m] / ws;
-        R yc = YC[m] / ws, ys = YS[m] / ws, ccv = CC[m] / ws, ssv = SS[m] / ws;
-        yc -= y_ * ct;
-        ys -= y_ * st;
-        ccv -= ct * ct;
-        ssv -= st * st;
-        yy -= y_ * y_;
-        const R pw = (yc * yc / ccv + ys * ys / ssv) / yy;
-        if (kf < a.nf) {
-            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-            peaks_insert(L, (double)pw, kf);
+        R *pg = static_cast<R *>(a.pgram);
+        Peaks L;
+        peaks_init(L);
+    #pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const uint32_t kf = kbase + m * T + tid;
+            const R ws = wsum[m];
+            R y_ = Y[m] / ws, yy = YY[m] / ws, ct = Ct[m] / ws, st = St[m] / ws;
+            R yc = YC[m] / ws, ys = YS[m] / ws, ccv = CC[m] / ws, ssv = SS[m] / ws;
+            yc -= y_ * ct;
+            ys -= y_ * st;
+            ccv -= ct * ct;
+            ssv -= st * st;
+            yy -= y_ * y_;
+            const R pw = (yc * yc / ccv + ys * ys / ssv) / yy;
+            if (kf < a.nf) {
+                if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+                peaks_insert(L, (double)pw, kf);
+            }
         }
+        if (a.want_peaks) write_partial<R, T>(a, L, obj, chunk, s_pp, s_pi);
     }
-    if (a.want_peaks) write_partial<R, T>(a, L, obj, chunk, s_pp, s_pi);
 }
 
 // ---------------------------------------------------------------------------
@@ -528,109 +557,113 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
     __shared__ float4 s_a[T];
     __shared__ double s_pp[T / 64][kPeaks];
     __shared__ uint32_t s_pi[T / 64][kPeaks];
-    uint32_t obj, chunk;
-    item_coords(a, obj, chunk);
-    const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t kbase = chunk * (uint32_t)(T * M);
-    const double fseed = a.fmin_rev + (double)(kbase + tid) * a.df_rev;
+    __shared__ uint64_t s_item;
+    for (uint64_t item = first_item(a, &s_item); item < a.total_items;
+         item = next_item(a, &s_item)) {
+        uint32_t obj, chunk;
+        item_coords(a, item, obj, chunk);
+        const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
+        const uint32_t tid = threadIdx.x;
+        const uint32_t kbase = chunk * (uint32_t)(T * M);
+        const double fseed = a.fmin_rev + (double)(kbase + tid) * a.df_rev;
 
-    double acc[NS][M];
-#pragma unroll
-    for (int q = 0; q < NS; ++q)
-#pragma unroll
-        for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
+        double acc[NS][M];
+    #pragma unroll
+        for (int q = 0; q < NS; ++q)
+    #pragma unroll
+            for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
 
-    for (uint64_t b0 = beg; b0 < end; b0 += T) {
-        const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
-        __syncthreads();
-        if (tid < cnt) {
-            s_t[tid] = a.rec_t[b0 + tid];
-            s_a[tid] = a.rec_a[b0 + tid];
-        }
-        __syncthreads();
-        for (uint32_t j0 = 0; j0 < cnt; j0 += kFlush) {
-            const uint32_t j1 = (j0 + kFlush < cnt) ? j0 + kFlush : cnt;
-            float p[NS][M];
-#pragma unroll
-            for (int q = 0; q < NS; ++q)
-#pragma unroll
-                for (int m = 0; m < M; ++m) p[q][m] = 0.0f;
-            for (uint32_t j = j0; j < j1; ++j) {
-                const double tj = s_t[j];
-                const float4 r = s_a[j];
-                double ph = fseed * tj;
-                ph -= __builtin_rint(ph);
-                const float phf = (float)ph;
-                float c = __builtin_amdgcn_cosf(phf);
-                float s = __builtin_amdgcn_sinf(phf);
-#pragma unroll
-                for (int m = 0; m < M; ++m) {
-                    if constexpr (GEN) {
-                        const float wc = r.y * c, ws = r.y * s;
-                        p[0][m] += wc;
-                        p[1][m] += ws;
-                        p[2][m] = fmaf(r.x, c, p[2][m]);
-                        p[3][m] = fmaf(r.x, s, p[3][m]);
-                        p[4][m] = fmaf(wc, c, p[4][m]);
-                        p[5][m] = fmaf(ws, s, p[5][m]);
-                        p[6][m] = fmaf(wc, s, p[6][m]);
-                    } else {
-                        p[0][m] = fmaf(r.x, c, p[0][m]);
-                        p[1][m] = fmaf(r.x, s, p[1][m]);
-                        p[2][m] = fmaf(c, c, p[2][m]);
-                        p[3][m] = fmaf(s, s, p[3][m]);
-                        p[4][m] = fmaf(c, s, p[4][m]);
-                    }
-                    if (m + 1 < M) {
-                        const float cn = fmaf(c, r.z, -s * r.w);
-                        const float sn = fmaf(s, r.z, c * r.w);
-                        c = cn;
-                        s = sn;
+        for (uint64_t b0 = beg; b0 < end; b0 += T) {
+            const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+            __syncthreads();
+            if (tid < cnt) {
+                s_t[tid] = a.rec_t[b0 + tid];
+                s_a[tid] = a.rec_a[b0 + tid];
+            }
+            __syncthreads();
+            for (uint32_t j0 = 0; j0 < cnt; j0 += kFlush) {
+                const uint32_t j1 = (j0 + kFlush < cnt) ? j0 + kFlush : cnt;
+                float p[NS][M];
+    #pragma unroll
+                for (int q = 0; q < NS; ++q)
+    #pragma unroll
+                    for (int m = 0; m < M; ++m) p[q][m] = 0.0f;
+                for (uint32_t j = j0; j < j1; ++j) {
+                    const double tj = s_t[j];
+                    const float4 r = s_a[j];
+                    double ph = fseed * tj;
+                    ph -= __builtin_rint(ph);
+                    const float phf = (float)ph;
+                    float c = __builtin_amdgcn_cosf(phf);
+                    float s = __builtin_amdgcn_sinf(phf);
+    #pragma unroll
+                    for (int m = 0; m < M; ++m) {
+                        if constexpr (GEN) {
+                            const float wc = r.y * c, ws = r.y * s;
+                            p[0][m] += wc;
+                            p[1][m] += ws;
+                            p[2][m] = fmaf(r.x, c, p[2][m]);
+                            p[3][m] = fmaf(r.x, s, p[3][m]);
+                            p[4][m] = fmaf(wc, c, p[4][m]);
+                            p[5][m] = fmaf(ws, s, p[5][m]);
+                            p[6][m] = fmaf(wc, s, p[6][m]);
+                        } else {
+                            p[0][m] = fmaf(r.x, c, p[0][m]);
+                            p[1][m] = fmaf(r.x, s, p[1][m]);
+                            p[2][m] = fmaf(c, c, p[2][m]);
+                            p[3][m] = fmaf(s, s, p[3][m]);
+                            p[4][m] = fmaf(c, s, p[4][m]);
+                        }
+                        if (m + 1 < M) {
+                            const float cn = fmaf(c, r.z, -s * r.w);
+                            const float sn = fmaf(s, r.z, c * r.w);
+                            c = cn;
+                            s = sn;
+                        }
                     }
                 }
+    #pragma unroll
+                for (int q = 0; q < NS; ++q)
+    #pragma unroll
+                    for (int m = 0; m < M; ++m) acc[q][m] += (double)p[q][m];
             }
-#pragma unroll
-            for (int q = 0; q < NS; ++q)
-#pragma unroll
-                for (int m = 0; m < M; ++m) acc[q][m] += (double)p[q][m];
         }
-    }
 
-    const double scale = a.obj_scale[obj];
-    double Wf = 1.0, Yb = 0.0, YY = 1.0;
-    if constexpr (GEN) {
-        Wf = a.obj_gen[kGen * obj + 0];
-        Yb = a.obj_gen[kGen * obj + 1];
-        YY = a.obj_gen[kGen * obj + 2];
-    }
-    OUT *pg = static_cast<OUT *>(a.pgram);
-    Peaks L;
-    peaks_init(L);
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        const uint32_t kf = kbase + m * T + tid;
-        double P;
+        const double scale = a.obj_scale[obj];
+        double Wf = 1.0, Yb = 0.0, YY = 1.0;
         if constexpr (GEN) {
-            const double C = acc[0][m] / Wf, S = acc[1][m] / Wf;
-            const double YC = acc[2][m] / Wf - Yb * C, YS = acc[3][m] / Wf - Yb * S;
-            const double CC = acc[4][m] / Wf - C * C, SS = acc[5][m] / Wf - S * S;
-            const double CS = acc[6][m] / Wf - C * S;
-            const double D = CC * SS - CS * CS;
-            P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
-        } else {
-            const double XC = acc[0][m], XS = acc[1][m];
-            const double CC = acc[2][m], SS = acc[3][m], CS = acc[4][m];
-            const double D = CC * SS - CS * CS;
-            P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+            Wf = a.obj_gen[kGen * obj + 0];
+            Yb = a.obj_gen[kGen * obj + 1];
+            YY = a.obj_gen[kGen * obj + 2];
         }
-        const OUT pw = (OUT)(P * scale);
-        if (kf < a.nf) {
-            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-            peaks_insert(L, (double)pw, kf);
+        OUT *pg = static_cast<OUT *>(a.pgram);
+        Peaks L;
+        peaks_init(L);
+    #pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const uint32_t kf = kbase + m * T + tid;
+            double P;
+            if constexpr (GEN) {
+                const double C = acc[0][m] / Wf, S = acc[1][m] / Wf;
+                const double YC = acc[2][m] / Wf - Yb * C, YS = acc[3][m] / Wf - Yb * S;
+                const double CC = acc[4][m] / Wf - C * C, SS = acc[5][m] / Wf - S * S;
+                const double CS = acc[6][m] / Wf - C * S;
+                const double D = CC * SS - CS * CS;
+                P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
+            } else {
+                const double XC = acc[0][m], XS = acc[1][m];
+                const double CC = acc[2][m], SS = acc[3][m], CS = acc[4][m];
+                const double D = CC * SS - CS * CS;
+                P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+            }
+            const OUT pw = (OUT)(P * scale);
+            if (kf < a.nf) {
+                if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+                peaks_insert(L, (double)pw, kf);
+            }
         }
+        if (a.want_peaks) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
     }
-    if (a.want_peaks) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
 }
 
 // ---------------------------------------------------------------------------
@@ -652,121 +685,125 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
     __shared__ float2 s_a[T];
     __shared__ double s_pp[T / 64][kPeaks];
     __shared__ uint32_t s_pi[T / 64][kPeaks];
-    uint32_t obj, chunk;
-    item_coords(a, obj, chunk);
-    const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t kbase = chunk * (uint32_t)(T * M);
-    const double fseed = a.fmin_rev + (double)(kbase + tid) * a.df_rev;
+    __shared__ uint64_t s_item;
+    for (uint64_t item = first_item(a, &s_item); item < a.total_items;
+         item = next_item(a, &s_item)) {
+        uint32_t obj, chunk;
+        item_coords(a, item, obj, chunk);
+        const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
+        const uint32_t tid = threadIdx.x;
+        const uint32_t kbase = chunk * (uint32_t)(T * M);
+        const double fseed = a.fmin_rev + (double)(kbase + tid) * a.df_rev;
 
-    double acc[NS][M];
-#pragma unroll
-    for (int q = 0; q < NS; ++q)
-#pragma unroll
-        for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
+        double acc[NS][M];
+    #pragma unroll
+        for (int q = 0; q < NS; ++q)
+    #pragma unroll
+            for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
 
-    for (uint64_t b0 = beg; b0 < end; b0 += T) {
-        const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
-        __syncthreads();
-        if (tid < cnt) {
-            s_t[tid] = a.rec_t[b0 + tid];
-            s_r[tid] = a.rec_r[b0 + tid];
-            const float4 r4 = a.rec_a[b0 + tid];
-            s_a[tid] = make_float2(r4.x, r4.y);
-        }
-        __syncthreads();
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const double tj = s_t[j];
-            const double2 st = s_r[j];
-            const float2 yw = s_a[j];
-            double ph = fseed * tj;
-            ph -= __builtin_rint(ph);
-            const float phf = (float)ph;
-            const float cf = __builtin_amdgcn_cosf(phf);
-            const float sf = __builtin_amdgcn_sinf(phf);
-            const double res = (ph - (double)phf) * kTwoPi;  // exact residual, radians
-            double c = fma(-res, (double)sf, (double)cf);
-            double s = fma(res, (double)cf, (double)sf);
-            const double yv = (double)yw.x;
-            const double wv = (double)yw.y;
-            auto accumulate = [&](int m, double cv, double sv) {
-                if constexpr (GEN) {
-                    const double wc = wv * cv, ws = wv * sv;
-                    acc[0][m] += wc;
-                    acc[1][m] += ws;
-                    acc[2][m] = fma(yv, cv, acc[2][m]);
-                    acc[3][m] = fma(yv, sv, acc[3][m]);
-                    acc[4][m] = fma(wc, cv, acc[4][m]);
-                    acc[5][m] = fma(ws, sv, acc[5][m]);
-                    acc[6][m] = fma(wc, sv, acc[6][m]);
-                } else {
-                    acc[0][m] = fma(yv, cv, acc[0][m]);
-                    acc[1][m] = fma(yv, sv, acc[1][m]);
-                    acc[2][m] = fma(cv, cv, acc[2][m]);
-                    acc[3][m] = fma(sv, sv, acc[3][m]);
-                    acc[4][m] = fma(cv, sv, acc[4][m]);
+        for (uint64_t b0 = beg; b0 < end; b0 += T) {
+            const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+            __syncthreads();
+            if (tid < cnt) {
+                s_t[tid] = a.rec_t[b0 + tid];
+                s_r[tid] = a.rec_r[b0 + tid];
+                const float4 r4 = a.rec_a[b0 + tid];
+                s_a[tid] = make_float2(r4.x, r4.y);
+            }
+            __syncthreads();
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const double tj = s_t[j];
+                const double2 st = s_r[j];
+                const float2 yw = s_a[j];
+                double ph = fseed * tj;
+                ph -= __builtin_rint(ph);
+                const float phf = (float)ph;
+                const float cf = __builtin_amdgcn_cosf(phf);
+                const float sf = __builtin_amdgcn_sinf(phf);
+                const double res = (ph - (double)phf) * kTwoPi;  // exact residual, radians
+                double c = fma(-res, (double)sf, (double)cf);
+                double s = fma(res, (double)cf, (double)sf);
+                const double yv = (double)yw.x;
+                const double wv = (double)yw.y;
+                auto accumulate = [&](int m, double cv, double sv) {
+                    if constexpr (GEN) {
+                        const double wc = wv * cv, ws = wv * sv;
+                        acc[0][m] += wc;
+                        acc[1][m] += ws;
+                        acc[2][m] = fma(yv, cv, acc[2][m]);
+                        acc[3][m] = fma(yv, sv, acc[3][m]);
+                        acc[4][m] = fma(wc, cv, acc[4][m]);
+                        acc[5][m] = fma(ws, sv, acc[5][m]);
+                        acc[6][m] = fma(wc, sv, acc[6][m]);
+                    } else {
+                        acc[0][m] = fma(yv, cv, acc[0][m]);
+                        acc[1][m] = fma(yv, sv, acc[1][m]);
+                        acc[2][m] = fma(cv, cv, acc[2][m]);
+                        acc[3][m] = fma(sv, sv, acc[3][m]);
+                        acc[4][m] = fma(cv, sv, acc[4][m]);
+                    }
+                };
+                // Bin 1 is bin 0 rotated by the observation's step D; bins m >= 2
+                // follow from the three-term recurrence z[m+1] = 2 cos(D) z[m] -
+                // z[m-1] (2 FMA per bin instead of 4).  In FP64 its error stays
+                // below 2e-14 over 16 bins for every D, including D near 0 and pi
+                // (the rounding of cos D only perturbs the angle by O(D^2) there).
+                const double twocd = 2.0 * st.x;
+                double cp = c, sp = s;
+                accumulate(0, c, s);
+                if constexpr (M > 1) {
+                    c = fma(cp, st.x, -sp * st.y);
+                    s = fma(sp, st.x, cp * st.y);
+                    accumulate(1, c, s);
                 }
-            };
-            // Bin 1 is bin 0 rotated by the observation's step D; bins m >= 2
-            // follow from the three-term recurrence z[m+1] = 2 cos(D) z[m] -
-            // z[m-1] (2 FMA per bin instead of 4).  In FP64 its error stays
-            // below 2e-14 over 16 bins for every D, including D near 0 and pi
-            // (the rounding of cos D only perturbs the angle by O(D^2) there).
-            const double twocd = 2.0 * st.x;
-            double cp = c, sp = s;
-            accumulate(0, c, s);
-            if constexpr (M > 1) {
-                c = fma(cp, st.x, -sp * st.y);
-                s = fma(sp, st.x, cp * st.y);
-                accumulate(1, c, s);
-            }
-#pragma unroll
-            for (int m = 2; m < M; ++m) {
-                const double cn = fma(twocd, c, -cp);
-                const double sn = fma(twocd, s, -sp);
-                cp = c;
-                sp = s;
-                c = cn;
-                s = sn;
-                accumulate(m, c, s);
+    #pragma unroll
+                for (int m = 2; m < M; ++m) {
+                    const double cn = fma(twocd, c, -cp);
+                    const double sn = fma(twocd, s, -sp);
+                    cp = c;
+                    sp = s;
+                    c = cn;
+                    s = sn;
+                    accumulate(m, c, s);
+                }
             }
         }
-    }
 
-    const double scale = a.obj_scale[obj];
-    double Wf = 1.0, Yb = 0.0, YY = 1.0;
-    if constexpr (GEN) {
-        Wf = a.obj_gen[kGen * obj + 0];
-        Yb = a.obj_gen[kGen * obj + 1];
-        YY = a.obj_gen[kGen * obj + 2];
-    }
-    OUT *pg = static_cast<OUT *>(a.pgram);
-    Peaks L;
-    peaks_init(L);
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        const uint32_t kf = kbase + m * T + tid;
-        double P;
+        const double scale = a.obj_scale[obj];
+        double Wf = 1.0, Yb = 0.0, YY = 1.0;
         if constexpr (GEN) {
-            const double C = acc[0][m] / Wf, S = acc[1][m] / Wf;
-            const double YC = acc[2][m] / Wf - Yb * C, YS = acc[3][m] / Wf - Yb * S;
-            const double CC = acc[4][m] / Wf - C * C, SS = acc[5][m] / Wf - S * S;
-            const double CS = acc[6][m] / Wf - C * S;
-            const double D = CC * SS - CS * CS;
-            P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
-        } else {
-            const double XC = acc[0][m], XS = acc[1][m];
-            const double CC = acc[2][m], SS = acc[3][m], CS = acc[4][m];
-            const double D = CC * SS - CS * CS;
-            P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+            Wf = a.obj_gen[kGen * obj + 0];
+            Yb = a.obj_gen[kGen * obj + 1];
+            YY = a.obj_gen[kGen * obj + 2];
         }
-        const OUT pw = (OUT)(P * scale);
-        if (kf < a.nf) {
-            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-            peaks_insert(L, (double)pw, kf);
+        OUT *pg = static_cast<OUT *>(a.pgram);
+        Peaks L;
+        peaks_init(L);
+    #pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const uint32_t kf = kbase + m * T + tid;
+            double P;
+            if constexpr (GEN) {
+                const double C = acc[0][m] / Wf, S = acc[1][m] / Wf;
+                const double YC = acc[2][m] / Wf - Yb * C, YS = acc[3][m] / Wf - Yb * S;
+                const double CC = acc[4][m] / Wf - C * C, SS = acc[5][m] / Wf - S * S;
+                const double CS = acc[6][m] / Wf - C * S;
+                const double D = CC * SS - CS * CS;
+                P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
+            } else {
+                const double XC = acc[0][m], XS = acc[1][m];
+                const double CC = acc[2][m], SS = acc[3][m], CS = acc[4][m];
+                const double D = CC * SS - CS * CS;
+                P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+            }
+            const OUT pw = (OUT)(P * scale);
+            if (kf < a.nf) {
+                if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+                peaks_insert(L, (double)pw, kf);
+            }
         }
+        if (a.want_peaks) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
     }
-    if (a.want_peaks) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
 }
 
 // ---------------------------------------------------------------------------
@@ -826,10 +863,17 @@ const TileEntry kTiles[] = {
     LS_SPLIT(1, float, LSAMD_F32, 256, 4), LS_SPLIT(1, double, LSAMD_F64, 256, 4),
     LS_SPLIT(1, float, LSAMD_F32, 256, 8), LS_SPLIT(1, double, LSAMD_F64, 256, 8),
     LS_SPLIT(1, float, LSAMD_F32, 128, 8), LS_SPLIT(1, double, LSAMD_F64, 128, 8),
+    LS_SPLIT(0, float, LSAMD_F32, 128, 16), LS_SPLIT(0, double, LSAMD_F64, 128, 16),
+    LS_SPLIT(1, float, LSAMD_F32, 256, 2), LS_SPLIT(1, double, LSAMD_F64, 256, 2),
     // split32 form (FP32 rotation and chunk sums)
     LS_SPLIT32(0, float, LSAMD_F32, 256, 8), LS_SPLIT32(0, double, LSAMD_F64, 256, 8),
     LS_SPLIT32(0, float, LSAMD_F32, 256, 4), LS_SPLIT32(0, double, LSAMD_F64, 256, 4),
+    LS_SPLIT32(0, float, LSAMD_F32, 256, 2), LS_SPLIT32(0, double, LSAMD_F64, 256, 2),
+    LS_SPLIT32(0, float, LSAMD_F32, 128, 4), LS_SPLIT32(0, double, LSAMD_F64, 128, 4),
+    LS_SPLIT32(0, float, LSAMD_F32, 512, 4), LS_SPLIT32(0, double, LSAMD_F64, 512, 4),
     LS_SPLIT32(1, float, LSAMD_F32, 256, 4), LS_SPLIT32(1, double, LSAMD_F64, 256, 4),
+    LS_SPLIT32(1, float, LSAMD_F32, 256, 2), LS_SPLIT32(1, double, LSAMD_F64, 256, 2),
+    LS_SPLIT32(1, float, LSAMD_F32, 128, 4), LS_SPLIT32(1, double, LSAMD_F64, 128, 4),
     // plain form
     LS_PLAIN_STD(float, LSAMD_F32, 256, 1), LS_PLAIN_STD(double, LSAMD_F64, 256, 1),
     LS_PLAIN_STD(float, LSAMD_F32, 256, 2), LS_PLAIN_STD(double, LSAMD_F64, 256, 2),
@@ -848,9 +892,13 @@ constexpr int kForms = 3;
 int g_tile_T[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
 int g_tile_M[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
 
+// Built-in shapes: the fastest of the tile sweep on the 1000-object x 1e6
+// batch (MI355X): split 256x16 (std, 42.9 ms) / 256x8 (gen, 70.7 ms);
+// split32 256x4 (std 37.6 ms, gen 53.0 ms).  lsamd_autotune re-picks per
+// workload.
 Tile default_tile(int form, int variant) {
-    if (form == LSAMD_FORM_SPLIT) return variant ? Tile{256, 4} : Tile{256, 8};
-    if (form == LSAMD_FORM_SPLIT32) return variant ? Tile{256, 4} : Tile{256, 8};
+    if (form == LSAMD_FORM_SPLIT) return variant ? Tile{256, 8} : Tile{256, 16};
+    if (form == LSAMD_FORM_SPLIT32) return Tile{256, 4};
     return Tile{256, 1};
 }
 
@@ -903,6 +951,7 @@ int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
     const size_t parts = (size_t)K * w->chunks * kPeaks;
     w->part_pow = take(8 * parts);
     w->part_idx = take(4 * parts);
+    w->queue = take(8);
     w->bytes = off + 256;
     return LSAMD_OK;
 }
@@ -980,6 +1029,8 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: max_freq must exceed min_freq");
     if (a->pgram == nullptr && a->top_k == 0)
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: nothing to compute (no pgram, top_k=0)");
+    if (a->schedule != LSAMD_SCHEDULE_GRID && a->schedule != LSAMD_SCHEDULE_PERSISTENT)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: bad schedule");
     Workspace w;
     plan_workspace(a, &w);
     if (!workspace || workspace_bytes < w.bytes)
@@ -1072,22 +1123,48 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     sw.want_peaks = a->top_k > 0 ? 1 : 0;
 
     const uint64_t items = (uint64_t)K * w.chunks;
-    const uint64_t max_items = (0xFFFFFFFFull / (uint64_t)T) - 1;
-    if (g_timing) {
+    sw.total_items = items;
+    sw.queue = nullptr;
+    const bool persistent = a->schedule == LSAMD_SCHEDULE_PERSISTENT;
+    uint32_t pgrid = 0;
+    if (persistent) {
+        sw.queue = reinterpret_cast<unsigned long long *>(ws + w.queue);
+        int dev = 0, cus = 0, per_cu = 0;
+        LS_HIP(hipGetDevice(&dev));
+        LS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(te->fn), T, 0));
+        pgrid = (uint32_t)std::max(1, cus * std::max(1, per_cu));
+        if ((uint64_t)pgrid > items) pgrid = (uint32_t)items;
+        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, sw.queue);
+        LS_HIP(hipGetLastError());
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    LS_HIP(hipStreamIsCapturing(st, &cap));
+    const bool timing = g_timing && cap == hipStreamCaptureStatusNone;
+    if (timing) {
         if (!g_ev[0]) {
             LS_HIP(hipEventCreate(&g_ev[0]));
             LS_HIP(hipEventCreate(&g_ev[1]));
         }
         LS_HIP(hipEventRecord(g_ev[0], st));
     }
-    for (uint64_t base = 0; base < items; base += max_items) {
-        const uint64_t cnt = std::min<uint64_t>(max_items, items - base);
-        sw.item_base = base;
+    if (persistent) {
+        sw.item_base = 0;
         void *kargs[] = {&sw};
-        LS_HIP(hipLaunchKernel(reinterpret_cast<const void *>(te->fn), dim3((uint32_t)cnt), dim3(T),
-                               kargs, 0, st));
+        LS_HIP(hipLaunchKernel(reinterpret_cast<const void *>(te->fn), dim3(pgrid), dim3(T), kargs,
+                               0, st));
+    } else {
+        const uint64_t max_items = (0xFFFFFFFFull / (uint64_t)T) - 1;
+        for (uint64_t base = 0; base < items; base += max_items) {
+            const uint64_t cnt = std::min<uint64_t>(max_items, items - base);
+            sw.item_base = base;
+            void *kargs[] = {&sw};
+            LS_HIP(hipLaunchKernel(reinterpret_cast<const void *>(te->fn), dim3((uint32_t)cnt),
+                                   dim3(T), kargs, 0, st));
+        }
     }
-    if (g_timing) {
+    if (timing) {
         LS_HIP(hipEventRecord(g_ev[1], st));
         g_ev_pending = true;
     }
@@ -1118,6 +1195,152 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
             hipLaunchKernelGGL(k_reduce_peaks<float>, dim3(K), dim3(64), 0, st, rd);
         LS_HIP(hipGetLastError());
     }
+    return LSAMD_OK;
+}
+
+struct lsamd_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int lsamd_graph_create(const lsamd_sweep_args *a, void *workspace, size_t workspace_bytes,
+                       void *stream_v, lsamd_graph **out) {
+    if (!out || !stream_v)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_graph_create: needs an output pointer and a "
+                                            "non-default stream");
+    *out = nullptr;
+    hipStream_t st = static_cast<hipStream_t>(stream_v);
+    LS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = lsamd_sweep(a, workspace, workspace_bytes, stream_v);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    lsamd_graph *g = new lsamd_graph();
+    if (rc != LSAMD_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        delete g;
+        return rc != LSAMD_OK ? rc
+                              : set_error(LSAMD_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                                             hipGetErrorString(e));
+    }
+    g->graph = graph;
+    const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        delete g;
+        return set_error(LSAMD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    *out = g;
+    return LSAMD_OK;
+}
+
+int lsamd_graph_launch(lsamd_graph *g, void *stream_v) {
+    if (!g || !g->exec) return set_error(LSAMD_ERR_INVALID, "lsamd_graph_launch: null graph");
+    hipStream_t st = static_cast<hipStream_t>(stream_v);
+    if (g_timing) {
+        if (!g_ev[0]) {
+            LS_HIP(hipEventCreate(&g_ev[0]));
+            LS_HIP(hipEventCreate(&g_ev[1]));
+        }
+        LS_HIP(hipEventRecord(g_ev[0], st));
+    }
+    LS_HIP(hipGraphLaunch(g->exec, st));
+    if (g_timing) {
+        LS_HIP(hipEventRecord(g_ev[1], st));
+        g_ev_pending = true;
+    }
+    return LSAMD_OK;
+}
+
+void lsamd_graph_destroy(lsamd_graph *g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
+// The autotuner's trial sweeps: every object, at most 65536 bins, peaks only.
+static lsamd_sweep_args autotune_sample(const lsamd_sweep_args *a) {
+    lsamd_sweep_args s = *a;
+    s.num_freqs = std::min<uint32_t>(a->num_freqs, 65536u);
+    s.max_freq = a->min_freq + (a->max_freq - a->min_freq) * ((double)s.num_freqs / a->num_freqs);
+    s.pgram = nullptr;
+    s.top_k = 1;
+    s.peak_index = nullptr;
+    s.peak_power = nullptr;
+    s.peak_period = nullptr;
+    return s;
+}
+
+size_t lsamd_autotune_workspace_size(const lsamd_sweep_args *a) {
+    if (!a || a->form < 0 || a->form >= kForms || a->variant < 0 || a->variant > 1) return 0;
+    const lsamd_sweep_args s = autotune_sample(a);
+    const int saved_T = g_tile_T[a->form][a->variant], saved_M = g_tile_M[a->form][a->variant];
+    size_t need = 0;
+    for (const TileEntry &te : kTiles) {
+        if (te.form != a->form || te.variant != a->variant || te.dtype != a->dtype) continue;
+        g_tile_T[a->form][a->variant] = te.T;
+        g_tile_M[a->form][a->variant] = te.M;
+        Workspace w;
+        plan_workspace(&s, &w);
+        need = std::max(need, w.bytes);
+    }
+    g_tile_T[a->form][a->variant] = saved_T;
+    g_tile_M[a->form][a->variant] = saved_M;
+    return need;
+}
+
+int lsamd_autotune(const lsamd_sweep_args *a, void *workspace, size_t workspace_bytes,
+                   void *stream_v, int *threads_out, int *fpt_out) {
+    if (!a) return set_error(LSAMD_ERR_INVALID, "lsamd_autotune: null args");
+    if (a->form < 0 || a->form >= kForms || a->variant < 0 || a->variant > 1)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_autotune: bad form/variant");
+    hipStream_t st = static_cast<hipStream_t>(stream_v);
+    lsamd_sweep_args s = autotune_sample(a);
+    const int form = a->form, variant = a->variant;
+    const int saved_T = g_tile_T[form][variant], saved_M = g_tile_M[form][variant];
+    const bool saved_timing = g_timing;
+    hipEvent_t e0, e1;
+    LS_HIP(hipEventCreate(&e0));
+    LS_HIP(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    int best_T = 0, best_M = 0;
+    for (const TileEntry &te : kTiles) {
+        if (te.form != form || te.variant != variant || te.dtype != a->dtype) continue;
+        g_tile_T[form][variant] = te.T;
+        g_tile_M[form][variant] = te.M;
+        Workspace w;
+        plan_workspace(&s, &w);
+        if (w.bytes > workspace_bytes) continue;
+        g_timing = false;
+        int rc = lsamd_sweep(&s, workspace, workspace_bytes, stream_v);  // warm-up
+        if (rc) break;
+        LS_HIP(hipEventRecord(e0, st));
+        rc = lsamd_sweep(&s, workspace, workspace_bytes, stream_v);
+        if (rc) break;
+        LS_HIP(hipEventRecord(e1, st));
+        LS_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        LS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_T = te.T;
+            best_M = te.M;
+        }
+    }
+    g_timing = saved_timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best_T == 0) {
+        g_tile_T[form][variant] = saved_T;
+        g_tile_M[form][variant] = saved_M;
+        return lsamd_last_error_code() ? lsamd_last_error_code()
+                                       : set_error(LSAMD_ERR_INVALID, "lsamd_autotune: no tile fits "
+                                                                      "the workspace");
+    }
+    g_tile_T[form][variant] = best_T;
+    g_tile_M[form][variant] = best_M;
+    if (threads_out) *threads_out = best_T;
+    if (fpt_out) *fpt_out = best_M;
     return LSAMD_OK;
 }
 

@@ -46,7 +46,7 @@ class SweepPlan:
 
     def __init__(self, t, y, dy, offsets, *, min_freq, max_freq, num_freqs, variant,
                  form=_native.FORM_SPLIT, normalize=True, precenter=True, pgram=True, top_k=1,
-                 order="longest_first", lib=None):
+                 order="longest_first", schedule=_native.SCHEDULE_GRID, lib=None):
         if not t.is_cuda:
             raise RuntimeError("SweepPlan needs CUDA (HIP) tensors")
         self.lib = lib or _native.core()
@@ -83,15 +83,63 @@ class SweepPlan:
             a.peak_index = self.peak_index.data_ptr()
             a.peak_power = self.peak_power.data_ptr()
             a.peak_period = self.peak_period.data_ptr()
+        a.schedule = int(schedule)
         self.args = a
         ws = int(self.lib.lsamd_sweep_workspace_size(byref(a)))
         self.workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
+        self.graph = None
+        self._graph_stream = None
+
+    def _stream(self, stream):
+        return stream if stream is not None else torch.cuda.current_stream(self.t.device)
+
+    def autotune(self, stream=None):
+        """Pick the fastest compiled tile shape for this plan's form/variant
+        (lsamd_autotune) and resize the workspace for it; returns (T, M)."""
+        from ctypes import c_int
+        need = int(self.lib.lsamd_autotune_workspace_size(byref(self.args)))
+        big = torch.empty(max(need, 256), dtype=torch.uint8, device=self.t.device)
+        T, M = c_int(), c_int()
+        rc = self.lib.lsamd_autotune(byref(self.args), c_void_p(big.data_ptr()), big.numel(),
+                                     c_void_p(self._stream(stream).cuda_stream), byref(T), byref(M))
+        _native.check(self.lib, rc)
+        del big
+        ws = int(self.lib.lsamd_sweep_workspace_size(byref(self.args)))
+        self.workspace = torch.empty(ws, dtype=torch.uint8, device=self.t.device)
+        self.destroy_graph()
+        return T.value, M.value
+
+    def capture_graph(self, stream):
+        """Capture this sweep into a HIP graph on `stream` (a non-default
+        torch.cuda.Stream); later run() calls replay it."""
+        self.destroy_graph()
+        g = c_void_p()
+        rc = self.lib.lsamd_graph_create(byref(self.args), c_void_p(self.workspace.data_ptr()),
+                                         self.workspace.numel(), c_void_p(stream.cuda_stream), byref(g))
+        _native.check(self.lib, rc)
+        self.graph = g
+        self._graph_stream = stream
+
+    def destroy_graph(self):
+        if self.graph is not None:
+            self.lib.lsamd_graph_destroy(self.graph)
+            self.graph = None
+
+    def __del__(self):
+        try:
+            self.destroy_graph()
+        except Exception:
+            pass
 
     def run(self, stream=None):
-        """Enqueue the sweep on `stream` (default: torch's current stream)."""
-        s = stream if stream is not None else torch.cuda.current_stream(self.t.device)
-        rc = self.lib.lsamd_sweep(byref(self.args), c_void_p(self.workspace.data_ptr()),
-                                  self.workspace.numel(), c_void_p(s.cuda_stream))
+        """Enqueue the sweep on `stream` (default: torch's current stream),
+        replaying the captured graph when there is one."""
+        s = self._stream(stream)
+        if self.graph is not None:
+            rc = self.lib.lsamd_graph_launch(self.graph, c_void_p(s.cuda_stream))
+        else:
+            rc = self.lib.lsamd_sweep(byref(self.args), c_void_p(self.workspace.data_ptr()),
+                                      self.workspace.numel(), c_void_p(s.cuda_stream))
         _native.check(self.lib, rc)
 
     def results(self):

@@ -80,6 +80,14 @@ enum lsamd_form {
      * bins, up to ~6e-5 at strongly aliased ones */
     LSAMD_FORM_SPLIT32 = 2
 };
+enum lsamd_schedule {
+    /* one workgroup per (object, frequency chunk) work item, in processing
+     * order; the hardware dispatcher balances the load */
+    LSAMD_SCHEDULE_GRID = 0,
+    /* persistent workgroups (as many as fit on the device) taking the next
+     * work item from a counter in the workspace until none are left */
+    LSAMD_SCHEDULE_PERSISTENT = 1
+};
 enum lsamd_status {
     LSAMD_OK = 0,
     LSAMD_ERR_INVALID = 1, /* bad argument */
@@ -115,6 +123,7 @@ typedef struct lsamd_sweep_args {
     uint32_t *peak_index;    /* device, K*top_k, or NULL */
     void *peak_power;        /* device, K*top_k, or NULL */
     void *peak_period;       /* device, K*top_k (days), or NULL */
+    int32_t schedule;        /* enum lsamd_schedule */
 } lsamd_sweep_args;
 
 /* Bytes of device workspace lsamd_sweep needs for these arguments. */
@@ -126,6 +135,24 @@ size_t lsamd_sweep_workspace_size(const lsamd_sweep_args *args);
  * not allocate. */
 int lsamd_sweep(const lsamd_sweep_args *args, void *workspace, size_t workspace_bytes,
                 void *stream);
+
+/* Capture one sweep (all its kernels, on a non-default stream) into a HIP
+ * graph for repeated launches with the same arguments and workspace; the
+ * capture clears its counters with kernels, not memset nodes. */
+typedef struct lsamd_graph lsamd_graph;
+int lsamd_graph_create(const lsamd_sweep_args *args, void *workspace, size_t workspace_bytes,
+                       void *stream, lsamd_graph **graph);
+int lsamd_graph_launch(lsamd_graph *graph, void *stream);
+void lsamd_graph_destroy(lsamd_graph *graph);
+
+/* Time every compiled tile shape of args->form/variant/dtype on these
+ * arguments (reduced to at most 65536 frequency bins, peaks only), keep the
+ * fastest for later sweeps (as lsamd_set_tile would) and report it.  Uses
+ * the caller's workspace; synchronises the stream. */
+int lsamd_autotune(const lsamd_sweep_args *args, void *workspace, size_t workspace_bytes,
+                   void *stream, int *threads, int *freqs_per_thread);
+/* Workspace bytes lsamd_autotune needs for these arguments. */
+size_t lsamd_autotune_workspace_size(const lsamd_sweep_args *args);
 
 /* Host-array batch: segmentation by runs of objectId, device staging,
  * lsamd_sweep, copy-back, all synchronous.  y is pre-centred IN PLACE for

@@ -324,7 +324,8 @@ def test_identical_times_zero_sum_gives_zero_power(form):
     r = _native.batch_host(None, t, y, None, 1.0, 5.0, 50, _native.STANDARD, F64, form,
                            normalize=False)
     g = r["pgram"][0]
-    assert np.all(~np.isfinite(g) | (np.abs(g) < 1e-9))
+    noise = 1e-6 if form == _native.FORM_SPLIT32 else 1e-9  # FP32 sums leave ~1e-8
+    assert np.all(~np.isfinite(g) | (np.abs(g) < noise))
 
 
 # --------------------------------------------------------------------------
@@ -355,3 +356,81 @@ def test_device_plan_matches_host_path():
     plan2.run()
     torch.cuda.synchronize()
     assert np.array_equal(plan2.results()["index"].cpu().numpy(), res["index"].cpu().numpy())
+
+
+# --------------------------------------------------------------------------
+# schedules, graph replay, autotuner
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT, _native.FORM_SPLIT32])
+def test_persistent_schedule_matches_grid_bitwise(form):
+    torch = pytest.importorskip("torch")
+    from gpu_lomb_scargle_amd import device
+    b = synth.lognormal_batch(300, 60, 1.0, 5, 1500, 1800, False, 44)
+    t, y, dy, off = device.numpy_to_device(b["obj_id"], b["t"], b["y"], None, F64)
+    kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=5003,
+              variant=_native.STANDARD, form=form, top_k=3)
+    g = device.SweepPlan(t, y, None, off, schedule=_native.SCHEDULE_GRID, **kw)
+    p = device.SweepPlan(t, y, None, off, schedule=_native.SCHEDULE_PERSISTENT, **kw)
+    g.run()
+    p.run()
+    torch.cuda.synchronize()
+    rg, rp = g.results(), p.results()
+    for k in ("pgram", "index", "power", "period"):
+        assert torch.equal(rg[k], rp[k]), k
+
+
+@pytest.mark.parametrize("schedule", [_native.SCHEDULE_GRID, _native.SCHEDULE_PERSISTENT])
+def test_graph_replays_match_eager(schedule):
+    """Replay the captured sweep several times with eager work and syncs in
+    between; every replay must equal the eager result (the capture clears its
+    queue counter with a kernel, not a memset node)."""
+    torch = pytest.importorskip("torch")
+    from gpu_lomb_scargle_amd import device
+    b = synth.lognormal_batch(120, 100, 0.8, 5, 1000, 2000, True, 45)
+    t, y, dy, off = device.numpy_to_device(b["obj_id"], b["t"], b["y"], b["dy"], F64)
+    kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=7001,
+              variant=_native.GENERALIZED, form=_native.FORM_SPLIT, top_k=3, schedule=schedule)
+    eager = device.SweepPlan(t, y, dy, off, **kw)
+    eager.run()
+    torch.cuda.synchronize()
+    ref = {k: v.clone() for k, v in eager.results().items()}
+    plan = device.SweepPlan(t, y, dy, off, **kw)
+    s = torch.cuda.Stream()
+    plan.capture_graph(s)
+    for rep in range(4):
+        plan.pgram.fill_(-1.0)
+        plan.peak_index.fill_(-1)
+        torch.cuda.synchronize()
+        plan.run(s)
+        s.synchronize()
+        out = plan.results()
+        for k in ("pgram", "index", "power", "period"):
+            assert torch.equal(out[k], ref[k]), (rep, k)
+        eager.run()  # eager work between replays
+        torch.cuda.synchronize()
+    plan.destroy_graph()
+
+
+def test_autotune_picks_a_compiled_tile():
+    torch = pytest.importorskip("torch")
+    from ctypes import byref, c_int
+    from gpu_lomb_scargle_amd import device
+    L = _native.core()
+    b = synth.lognormal_batch(200, 140, 0.6, 20, 1000, 1800, False, 46)
+    t, y, dy, off = device.numpy_to_device(b["obj_id"], b["t"], b["y"], None, F64)
+    try:
+        plan = device.SweepPlan(t, y, None, off, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
+                                num_freqs=40000, variant=_native.STANDARD, form=_native.FORM_SPLIT32,
+                                top_k=1)
+        T, M = plan.autotune()
+        Tg, Mg = c_int(), c_int()
+        L.lsamd_get_tile(_native.FORM_SPLIT32, _native.STANDARD, _native.F64, byref(Tg), byref(Mg))
+        assert (T, M) == (Tg.value, Mg.value) and T in (128, 256, 512) and M in (1, 2, 4, 8, 16)
+        plan.run()
+        torch.cuda.synchronize()
+        ref = oracle.batch(b["obj_id"][:b["counts"][:20].sum()], b["t"][:b["counts"][:20].sum()],
+                           b["y"][:b["counts"][:20].sum()], None, synth.OMEGA_MIN, synth.OMEGA_MAX,
+                           40000, False, F64, nthreads=8)
+        assert_close(plan.results()["pgram"][:20].cpu().numpy(), ref[0], 1e-5, 1e-3, 2e-6)
+    finally:
+        L.lsamd_set_tile(_native.FORM_SPLIT32, _native.STANDARD, 0, 0)
