@@ -37,7 +37,7 @@ EXPORTED = [
     "lsamd_last_error", "lsamd_clear_error", "lsamd_device_count", "lsamd_set_timing",
     "lsamd_last_sweep_kernel_ms", "lsamd_set_tile", "lsamd_get_tile", "lsamd_version",
     "lsamd_graph_create", "lsamd_graph_launch", "lsamd_graph_destroy", "lsamd_autotune",
-    "lsamd_autotune_workspace_size",
+    "lsamd_autotune_workspace_size", "lsamd_power_grad", "lsamd_refine_peaks",
 ]
 
 
@@ -60,6 +60,17 @@ class SweepArgs(Structure):
         ("offsets", c_void_p), ("order", c_void_p), ("t", c_void_p), ("y", c_void_p),
         ("dy", c_void_p), ("pgram", c_void_p), ("peak_index", c_void_p), ("peak_power", c_void_p),
         ("peak_period", c_void_p), ("schedule", c_int32),
+    ]
+
+
+class PointArgs(Structure):
+    """Mirror of ``lsamd_point_args``."""
+    _fields_ = [
+        ("variant", c_int32), ("dtype", c_int32), ("normalize", c_int32), ("num_objects", c_uint32),
+        ("num_obs", c_uint64), ("offsets", c_void_p), ("t", c_void_p), ("y", c_void_p),
+        ("dy", c_void_p), ("omega", c_void_p), ("power", c_void_p), ("dpower", c_void_p),
+        ("d2power", c_void_p), ("grad_scale", c_void_p), ("grad_y", c_void_p), ("grad_dy", c_void_p),
+        ("max_step", c_double), ("tol", c_double), ("max_iter", c_int32), ("iterations", c_void_p),
     ]
 
 
@@ -102,6 +113,10 @@ def _declare(L):
     L.lsamd_autotune.restype = c_int
     L.lsamd_autotune_workspace_size.argtypes = [POINTER(SweepArgs)]
     L.lsamd_autotune_workspace_size.restype = c_size_t
+    L.lsamd_power_grad.argtypes = [POINTER(PointArgs), c_void_p]
+    L.lsamd_power_grad.restype = c_int
+    L.lsamd_refine_peaks.argtypes = [POINTER(PointArgs), c_void_p]
+    L.lsamd_refine_peaks.restype = c_int
     return L
 
 

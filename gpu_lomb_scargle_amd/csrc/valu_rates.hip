@@ -6,7 +6,9 @@
 // device: FP32/FP64 FMA, FP32 multiply, v_sin_f32 / v_cos_f32 (input in
 // revolutions), and a sin + 4 FMA mix, each with 16-32 independent chains per
 // thread and 32 waves per CU so that issue, not latency, is the limit.
-// Output: lane-instructions per CU-cycle at the device's peak clock.
+// Output: lane-instructions per CU-cycle at the device's peak clock (for the
+// sweep-body kernel: bins per CU-cycle; its instruction count comes from the
+// SQ_INSTS_VALU counter).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -30,6 +32,12 @@ __global__ __launch_bounds__(256) void k_rate(float *out, double *outd, int iter
     for (int c = 0; c < 32; ++c) x[c] = 1e-3f * (threadIdx.x + c);
 #pragma unroll
     for (int c = 0; c < 16; ++c) d[c] = 1e-3 * (threadIdx.x + c) + blockIdx.x;
+    float u4[4], v4[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        u4[c] = 0.5f + 1e-4f * (threadIdx.x + c);
+        v4[c] = 0.25f - 1e-5f * (threadIdx.x + 3 * c);
+    }
     for (int i = 0; i < iters; ++i) {
         if constexpr (OP == 0) {
 #pragma unroll
@@ -53,6 +61,43 @@ __global__ __launch_bounds__(256) void k_rate(float *out, double *outd, int iter
         if constexpr (OP == 4) {
 #pragma unroll
             for (int c = 0; c < 32; ++c) x[c] = x[c] * a;
+        }
+        if constexpr (OP == 5) {
+            // accumulator form (two-address v_fmac_f32): x[c] += u[c%4] * v[c%4]
+#pragma unroll
+            for (int c = 0; c < 32; ++c) x[c] = fmaf(u4[c & 3], v4[c & 3], x[c]);
+        }
+        if constexpr (OP == 6) {
+            // 16 FP32 FMAs and 4 FP64 FMAs, independent
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x[c] = fmaf(u4[c & 3], v4[c & 3], x[c]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) d[c] = fma(d[c], 0.999, 0.25);
+        }
+        if constexpr (OP == 7) {
+            // the split32 sweep's per-observation body for 4 bins (standard):
+            // FP64 phase reduction, hardware cos/sin, 5 FMA sums per bin and a
+            // 4-instruction FP32 rotation between bins
+            const double tj = d[i & 15] + (double)i;
+            double ph = d[0] * 1e-3 * tj;
+            ph -= __builtin_rint(ph);
+            const float phf = (float)ph;
+            float c = __builtin_amdgcn_cosf(phf), s = __builtin_amdgcn_sinf(phf);
+            const float yv = u4[i & 3], rc = v4[0], rs = v4[1];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                x[5 * m + 0] = fmaf(yv, c, x[5 * m + 0]);
+                x[5 * m + 1] = fmaf(yv, s, x[5 * m + 1]);
+                x[5 * m + 2] = fmaf(c, c, x[5 * m + 2]);
+                x[5 * m + 3] = fmaf(s, s, x[5 * m + 3]);
+                x[5 * m + 4] = fmaf(c, s, x[5 * m + 4]);
+                if (m < 3) {
+                    const float cn = fmaf(c, rc, -s * rs);
+                    const float sn = fmaf(s, rc, c * rs);
+                    c = cn;
+                    s = sn;
+                }
+            }
         }
     }
     float s = 0.0f;
@@ -112,6 +157,11 @@ int main() {
     if (rate<1>("v_fma_f64 (16 chains)", 16, blocks, o, od, clock_hz, cus)) return 1;
     if (rate<2>("v_sin_f32 (32 chains)", 32, blocks, o, od, clock_hz, cus)) return 1;
     if (rate<3>("v_sin_f32 + 4 v_fma_f32", 160, blocks, o, od, clock_hz, cus)) return 1;
+    if (rate<5>("v_fmac_f32 (32 accumulators)", 32, blocks, o, od, clock_hz, cus)) return 1;
+    if (rate<6>("16 v_fmac_f32 + 4 v_fma_f64", 20, blocks, o, od, clock_hz, cus)) return 1;
+    // sweep body: 2 trans + 20 FMA sums + 12 rotation ops + ~8 phase/convert ops;
+    // counted as 1 term = 1/4 of the body, rate in lane-instructions is from the counters
+    if (rate<7>("sweep body, 4 bins (x 1/4)", 1, blocks, o, od, clock_hz, cus)) return 1;
 
     // accuracy of v_sin_f32 / v_cos_f32 on [-0.5, 0.5) revolutions against FP64
     const int n = 1 << 22;

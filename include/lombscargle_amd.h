@@ -154,6 +154,49 @@ int lsamd_autotune(const lsamd_sweep_args *args, void *workspace, size_t workspa
 /* Workspace bytes lsamd_autotune needs for these arguments. */
 size_t lsamd_autotune_workspace_size(const lsamd_sweep_args *args);
 
+/* ------------------------------------------------------------------------
+ * Power at arbitrary frequencies, its derivatives, and peak refinement.
+ * The reference has no counterpart: it only evaluates the power on its
+ * frequency grid (kernel.cu:484-631, main.cu:1566-1777).  The power is the
+ * same function of (omega, t, y, dy) as the reference's -- standard
+ * (normalised by 2/(N sigma_pop^2) when `normalize` is set, as its Python
+ * builds do) or generalized (AstroPy floating mean, weights dy^-2) -- and is
+ * evaluated in FP64 whatever the input dtype.  One wavefront per object.
+ * ---------------------------------------------------------------------- */
+typedef struct lsamd_point_args {
+    int32_t variant;         /* enum lsamd_variant */
+    int32_t dtype;           /* enum lsamd_dtype of t, y, dy, grad_y, grad_dy */
+    int32_t normalize;       /* standard: scale by 2/(N sigma_pop^2) */
+    uint32_t num_objects;    /* K >= 1 */
+    uint64_t num_obs;        /* n = offsets[K] */
+    const uint64_t *offsets; /* device, K+1 row offsets (each object >= 3 rows) */
+    const void *t, *y, *dy;  /* device, n each (dy: generalized only) */
+    double *omega;           /* device, K angular frequencies (rad/day); refined in
+                              * place by lsamd_refine_peaks */
+    double *power;           /* device, K, or NULL: P(omega) */
+    double *dpower;          /* device, K, or NULL: dP/domega */
+    double *d2power;         /* device, K, or NULL: d2P/domega2 */
+    const double *grad_scale;/* device, K, or NULL (= 1): upstream dL/dP per object */
+    void *grad_y;            /* device, n, or NULL: grad_scale[k] * dP/dy_j */
+    void *grad_dy;           /* device, n, or NULL: grad_scale[k] * dP/d(dy_j) (generalized) */
+    /* lsamd_refine_peaks only */
+    double max_step;         /* > 0: bound on |step| per Newton iteration (rad/day) */
+    double tol;              /* > 0: stop once |step| < tol */
+    int32_t max_iter;        /* >= 1 */
+    int32_t *iterations;     /* device, K, or NULL: Newton steps taken (negative: not converged) */
+} lsamd_point_args;
+
+/* Enqueue: power, dP/domega, d2P/domega2 at omega[k] and, when grad_y /
+ * grad_dy are given, the per-observation gradients scaled by grad_scale.
+ * dP/dy includes the dependence of the standard normalisation on y. */
+int lsamd_power_grad(const lsamd_point_args *args, void *stream);
+
+/* Enqueue: Newton iterations omega <- omega - P'/P'' per object (step bounded
+ * by max_step and taken uphill when P'' >= 0), until |step| < tol or max_iter;
+ * then power / dpower / d2power at the final omega.  Starting from the grid
+ * argmax, this moves each peak below the grid spacing. */
+int lsamd_refine_peaks(const lsamd_point_args *args, void *stream);
+
 /* Host-array batch: segmentation by runs of objectId, device staging,
  * lsamd_sweep, copy-back, all synchronous.  y is pre-centred IN PLACE for
  * the generalized variant when precenter is set (reference behaviour).

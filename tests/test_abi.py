@@ -86,6 +86,29 @@ def test_sweep_argument_checks():
     assert L.lsamd_sweep(byref(a), c_void_p(4096), need, None) == _native.ERR_INVALID
 
 
+def test_point_argument_checks():
+    """lsamd_power_grad / lsamd_refine_peaks validate before any launch."""
+    L = _native.core()
+    assert L.lsamd_power_grad(None, None) == _native.ERR_INVALID
+    a = _native.PointArgs()
+    assert L.lsamd_power_grad(byref(a), None) == _native.ERR_INVALID
+    assert b"empty" in L.lsamd_last_error()
+    a.num_objects, a.num_obs = 1, 10
+    a.offsets = a.t = a.y = a.omega = 1  # never dereferenced: rejected before any launch
+    assert L.lsamd_power_grad(byref(a), None) == _native.ERR_INVALID
+    assert b"nothing to compute" in L.lsamd_last_error()
+    a.variant = 1  # generalized without dy
+    a.power = 1
+    assert L.lsamd_power_grad(byref(a), None) == _native.ERR_INVALID
+    assert b"dy" in L.lsamd_last_error()
+    a.variant = 0
+    a.max_step, a.tol, a.max_iter = 0.0, 1e-9, 10
+    assert L.lsamd_refine_peaks(byref(a), None) == _native.ERR_INVALID
+    assert b"max_step" in L.lsamd_last_error()
+    a.dtype = 7
+    assert L.lsamd_power_grad(byref(a), None) == _native.ERR_INVALID
+
+
 def test_tile_registry():
     L = _native.core()
     assert L.lsamd_set_tile(1, 0, 256, 4) == _native.OK
