@@ -37,7 +37,8 @@ EXPORTED = [
     "lsamd_last_error", "lsamd_clear_error", "lsamd_device_count", "lsamd_set_timing",
     "lsamd_last_sweep_kernel_ms", "lsamd_set_tile", "lsamd_get_tile", "lsamd_version",
     "lsamd_graph_create", "lsamd_graph_launch", "lsamd_graph_destroy", "lsamd_autotune",
-    "lsamd_autotune_workspace_size", "lsamd_power_grad", "lsamd_refine_peaks",
+    "lsamd_autotune_workspace_size", "lsamd_power_grad", "lsamd_refine_peaks", "lsamd_read_table",
+    "lsamd_write_periods",
 ]
 
 
@@ -117,6 +118,11 @@ def _declare(L):
     L.lsamd_power_grad.restype = c_int
     L.lsamd_refine_peaks.argtypes = [POINTER(PointArgs), c_void_p]
     L.lsamd_refine_peaks.restype = c_int
+    L.lsamd_read_table.argtypes = [c_char_p, c_int, c_int, c_uint64, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, POINTER(c_uint64)]
+    L.lsamd_read_table.restype = c_int
+    L.lsamd_write_periods.argtypes = [c_char_p, c_void_p, c_uint32, c_uint32, c_int, c_void_p, c_void_p]
+    L.lsamd_write_periods.restype = c_int
     return L
 
 
@@ -198,3 +204,33 @@ def batch_host(obj_id, t, y, dy, fmin, fmax, nf, variant, dtype=np.float64, form
     check(L, rc)
     return {"pgram": pg, "index": idx[:, :tk], "power": pw[:, :tk], "period": per[:, :tk], "y": y,
             "K": int(kout.value)}
+
+
+def read_table(path, dtype=np.float64, with_dy=False, lib=None):
+    """Observation table in the reference's input format (lsamd_read_table):
+    returns (objectId uint32, t, y, dy or None) arrays of dtype."""
+    L = lib or core()
+    code = dtype_code(dtype)
+    n = c_uint64(0)
+    check(L, L.lsamd_read_table(os.fsencode(path), code, int(with_dy), 0, None, None, None, None,
+                                ctypes.byref(n)))
+    rows = n.value
+    ids = np.zeros(rows, np.uint32)
+    t, y = np.zeros(rows, dtype), np.zeros(rows, dtype)
+    dy = np.zeros(rows, dtype) if with_dy else None
+    check(L, L.lsamd_read_table(os.fsencode(path), code, int(with_dy), rows, ptr(ids), ptr(t), ptr(y),
+                                ptr(dy), ctypes.byref(n)))
+    return ids, t, y, dy
+
+
+def write_periods(path, object_ids, periods, powers, lib=None):
+    """Best periods in the reference's file format (lsamd_write_periods):
+    periods/powers of shape [K] (bestperiods.txt) or [K, 3]
+    (bestperiods_top_three.txt)."""
+    L = lib or core()
+    per = np.ascontiguousarray(periods)
+    pw = np.ascontiguousarray(powers, per.dtype)
+    ids = np.ascontiguousarray(object_ids, np.uint32)
+    top_k = 1 if per.ndim == 1 else per.shape[1]
+    check(L, L.lsamd_write_periods(os.fsencode(path), ptr(ids), ids.size, top_k, dtype_code(per.dtype),
+                                   ptr(per), ptr(pw)))

@@ -155,7 +155,8 @@ struct RecArgs {
     const double *obj_shift, *obj_gen;
     void *rec_y, *rec_w;       // plain
     double *rec_t;             // split
-    float4 *rec_a;             // split: {y or w*y, w, cos step, sin step} (FP32)
+    float4 *rec_a;             // split: {y or w*y, w or -sin step, cos step, sin step} (FP32)
+    float *rec_n;              // split generalized: -sin step (FP32)
     double2 *rec_r;            // split: {cos step, sin step} (FP64)
     double step_rev;           // split: T * df (revolutions/day per T bins)
     int precenter;
@@ -174,6 +175,7 @@ struct SweepArgs {
     // split
     const double *rec_t;
     const float4 *rec_a;
+    const float *rec_n;
     const double2 *rec_r;
     double fmin_rev, df_rev;   // revolutions/day
     const double *obj_gen;
@@ -293,14 +295,17 @@ __global__ void k_records_split(RecArgs a) {
         ps -= rint(ps);
         double sv, cv;
         sincospi(2.0 * ps, &sv, &cv);
+        // The FP32 rotation needs both sin D and -sin D as plain operands (an
+        // all-VGPR v_fmac_f32 issues at twice the rate of a negated VOP3 form).
         float4 r;
         if (GEN) {
             const double wi = 1.0 / ((double)dy[i] * (double)dy[i]) / W;
             r.x = (float)(wi * ((double)y[i] - ybar));
             r.y = (float)wi;
+            a.rec_n[i] = -(float)sv;
         } else {
             r.x = (float)y[i];
-            r.y = 1.0f;
+            r.y = -(float)sv;
         }
         r.z = (float)cv;
         r.w = (float)sv;
@@ -555,6 +560,7 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
     constexpr int NS = GEN ? 7 : 5;
     __shared__ double s_t[T];
     __shared__ float4 s_a[T];
+    __shared__ float s_n[GEN ? T : 1];
     __shared__ double s_pp[T / 64][kPeaks];
     __shared__ uint32_t s_pi[T / 64][kPeaks];
     __shared__ uint64_t s_item;
@@ -579,6 +585,7 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
             if (tid < cnt) {
                 s_t[tid] = a.rec_t[b0 + tid];
                 s_a[tid] = a.rec_a[b0 + tid];
+                if constexpr (GEN) s_n[tid] = a.rec_n[b0 + tid];
             }
             __syncthreads();
             for (uint32_t j0 = 0; j0 < cnt; j0 += kFlush) {
@@ -588,9 +595,11 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
                 for (int q = 0; q < NS; ++q)
     #pragma unroll
                     for (int m = 0; m < M; ++m) p[q][m] = 0.0f;
+    #pragma unroll 2
                 for (uint32_t j = j0; j < j1; ++j) {
                     const double tj = s_t[j];
                     const float4 r = s_a[j];
+                    const float nsd = GEN ? s_n[j] : r.y;  // -sin D
                     double ph = fseed * tj;
                     ph -= __builtin_rint(ph);
                     const float phf = (float)ph;
@@ -615,7 +624,7 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
                             p[4][m] = fmaf(c, s, p[4][m]);
                         }
                         if (m + 1 < M) {
-                            const float cn = fmaf(c, r.z, -s * r.w);
+                            const float cn = fmaf(c, r.z, s * nsd);
                             const float sn = fmaf(s, r.z, c * r.w);
                             c = cn;
                             s = sn;
@@ -672,14 +681,15 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
 // FP64-reduced phase, corrected to first order by the FP64 residual of that
 // phase's rounding to FP32; the M bins then follow in FP64 from the
 // observation's FP64 step (one rotation, then the three-term recurrence) and
-// accumulate directly in FP64.  On gfx950 FP64
-// FMA issues at the FP32 rate, so this costs ~10% more instructions than the
-// FP32-sum form and removes the FP32 summation error that dominates at
-// ill-conditioned (aliasing) frequencies.
+// accumulate directly in FP64.  This removes the FP32 summation error that
+// dominates at ill-conditioned (aliasing) frequencies.  The seed is brought
+// to unit modulus by one Newton step, so sum c^2 + sum s^2 equals the sum of
+// the weights to FP64 rounding and SS is not accumulated: SS = N - CC
+// (standard) or W - CC (generalized, normalised weights).
 // ---------------------------------------------------------------------------
 template <bool GEN, typename OUT, int T, int M>
 __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
-    constexpr int NS = GEN ? 7 : 5;
+    constexpr int NS = GEN ? 6 : 4;
     __shared__ double s_t[T];
     __shared__ double2 s_r[T];
     __shared__ float2 s_a[T];
@@ -723,24 +733,27 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
                 const double res = (ph - (double)phf) * kTwoPi;  // exact residual, radians
                 double c = fma(-res, (double)sf, (double)cf);
                 double s = fma(res, (double)cf, (double)sf);
+                // one Newton step towards |z| = 1: the hardware sin/cos leave
+                // |z|^2 - 1 ~ 1e-7, which SS = W - CC would otherwise carry
+                const double nf = fma(-0.5, fma(c, c, s * s), 1.5);
+                c *= nf;
+                s *= nf;
                 const double yv = (double)yw.x;
                 const double wv = (double)yw.y;
                 auto accumulate = [&](int m, double cv, double sv) {
                     if constexpr (GEN) {
-                        const double wc = wv * cv, ws = wv * sv;
+                        const double wc = wv * cv;
                         acc[0][m] += wc;
-                        acc[1][m] += ws;
+                        acc[1][m] = fma(wv, sv, acc[1][m]);
                         acc[2][m] = fma(yv, cv, acc[2][m]);
                         acc[3][m] = fma(yv, sv, acc[3][m]);
                         acc[4][m] = fma(wc, cv, acc[4][m]);
-                        acc[5][m] = fma(ws, sv, acc[5][m]);
-                        acc[6][m] = fma(wc, sv, acc[6][m]);
+                        acc[5][m] = fma(wc, sv, acc[5][m]);
                     } else {
                         acc[0][m] = fma(yv, cv, acc[0][m]);
                         acc[1][m] = fma(yv, sv, acc[1][m]);
                         acc[2][m] = fma(cv, cv, acc[2][m]);
-                        acc[3][m] = fma(sv, sv, acc[3][m]);
-                        acc[4][m] = fma(cv, sv, acc[4][m]);
+                        acc[3][m] = fma(cv, sv, acc[3][m]);
                     }
                 };
                 // Bin 1 is bin 0 rotated by the observation's step D; bins m >= 2
@@ -770,6 +783,7 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
         }
 
         const double scale = a.obj_scale[obj];
+        const double nobs = (double)(end - beg);
         double Wf = 1.0, Yb = 0.0, YY = 1.0;
         if constexpr (GEN) {
             Wf = a.obj_gen[kGen * obj + 0];
@@ -786,13 +800,14 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
             if constexpr (GEN) {
                 const double C = acc[0][m] / Wf, S = acc[1][m] / Wf;
                 const double YC = acc[2][m] / Wf - Yb * C, YS = acc[3][m] / Wf - Yb * S;
-                const double CC = acc[4][m] / Wf - C * C, SS = acc[5][m] / Wf - S * S;
-                const double CS = acc[6][m] / Wf - C * S;
+                const double CCw = acc[4][m] / Wf;
+                const double CC = CCw - C * C, SS = (1.0 - CCw) - S * S;
+                const double CS = acc[5][m] / Wf - C * S;
                 const double D = CC * SS - CS * CS;
                 P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
             } else {
                 const double XC = acc[0][m], XS = acc[1][m];
-                const double CC = acc[2][m], SS = acc[3][m], CS = acc[4][m];
+                const double CC = acc[2][m], CS = acc[3][m], SS = nobs - CC;
                 const double D = CC * SS - CS * CS;
                 P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
             }
@@ -874,6 +889,11 @@ const TileEntry kTiles[] = {
     LS_SPLIT32(1, float, LSAMD_F32, 256, 4), LS_SPLIT32(1, double, LSAMD_F64, 256, 4),
     LS_SPLIT32(1, float, LSAMD_F32, 256, 2), LS_SPLIT32(1, double, LSAMD_F64, 256, 2),
     LS_SPLIT32(1, float, LSAMD_F32, 128, 4), LS_SPLIT32(1, double, LSAMD_F64, 128, 4),
+    LS_SPLIT32(0, float, LSAMD_F32, 256, 6), LS_SPLIT32(0, double, LSAMD_F64, 256, 6),
+    LS_SPLIT32(1, float, LSAMD_F32, 256, 3), LS_SPLIT32(1, double, LSAMD_F64, 256, 3),
+    LS_SPLIT(0, float, LSAMD_F32, 256, 12), LS_SPLIT(0, double, LSAMD_F64, 256, 12),
+    LS_SPLIT32(0, float, LSAMD_F32, 128, 6), LS_SPLIT32(0, double, LSAMD_F64, 128, 6),
+    LS_SPLIT32(1, float, LSAMD_F32, 128, 3), LS_SPLIT32(1, double, LSAMD_F64, 128, 3),
     // plain form
     LS_PLAIN_STD(float, LSAMD_F32, 256, 1), LS_PLAIN_STD(double, LSAMD_F64, 256, 1),
     LS_PLAIN_STD(float, LSAMD_F32, 256, 2), LS_PLAIN_STD(double, LSAMD_F64, 256, 2),
@@ -893,12 +913,12 @@ int g_tile_T[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
 int g_tile_M[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
 
 // Built-in shapes: the fastest of the tile sweep on the 1000-object x 1e6
-// batch (MI355X): split 256x16 (std, 42.9 ms) / 256x8 (gen, 70.7 ms);
-// split32 256x4 (std 37.6 ms, gen 53.0 ms).  lsamd_autotune re-picks per
+// batches (MI355X): split 256x16 (standard) / 256x8 (generalized); split32
+// 256x6 (standard) / 128x4 (generalized).  lsamd_autotune re-picks per
 // workload.
 Tile default_tile(int form, int variant) {
     if (form == LSAMD_FORM_SPLIT) return variant ? Tile{256, 8} : Tile{256, 16};
-    if (form == LSAMD_FORM_SPLIT32) return Tile{256, 4};
+    if (form == LSAMD_FORM_SPLIT32) return variant ? Tile{128, 4} : Tile{256, 6};
     return Tile{256, 1};
 }
 
@@ -946,6 +966,7 @@ int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
     } else {
         w->rec_t = take(8 * n);
         w->rec_a = take(16 * n);
+        w->rec_n = take(4 * n);
         w->rec_r = take(16 * n);
     }
     const size_t parts = (size_t)K * w->chunks * kPeaks;
@@ -1076,6 +1097,7 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
         ra.rec_t = reinterpret_cast<double *>(ws + w.rec_t);
         ra.rec_a = reinterpret_cast<float4 *>(ws + w.rec_a);
         ra.rec_r = reinterpret_cast<double2 *>(ws + w.rec_r);
+        ra.rec_n = reinterpret_cast<float *>(ws + w.rec_n);
         ra.step_rev = (double)T * dw * kInv2Pi;
         void (*fn)(RecArgs) = f64 ? (gen ? k_records_split<double, true> : k_records_split<double, false>)
                                   : (gen ? k_records_split<float, true> : k_records_split<float, false>);
@@ -1113,6 +1135,7 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     sw.rec_t = split ? reinterpret_cast<const double *>(ws + w.rec_t) : nullptr;
     sw.rec_a = split ? reinterpret_cast<const float4 *>(ws + w.rec_a) : nullptr;
     sw.rec_r = split ? reinterpret_cast<const double2 *>(ws + w.rec_r) : nullptr;
+    sw.rec_n = split ? reinterpret_cast<const float *>(ws + w.rec_n) : nullptr;
     sw.fmin_rev = a->min_freq * kInv2Pi;
     sw.df_rev = dw * kInv2Pi;
     sw.obj_gen = obj_gen;

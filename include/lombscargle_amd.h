@@ -223,6 +223,23 @@ uint32_t lsamd_count_objects(const uint32_t *objectId, uint64_t n);
 uint32_t lsamd_object_offsets(const uint32_t *objectId, uint64_t n, uint64_t *offsets,
                               uint32_t *ids);
 
+/* Observation table in the reference's input format (importObjXYData,
+ * source/main.cu:1781-1843): numbers separated by commas and/or whitespace,
+ * rows of objectId, t, y (and dy when with_dy), each value parsed as dtype
+ * (LSAMD_F32 / LSAMD_F64) and the id converted from it.  With objectId NULL
+ * only *rows is set (size query); otherwise the arrays (capacity rows each)
+ * are filled. */
+int lsamd_read_table(const char *path, int dtype, int with_dy, uint64_t capacity,
+                     uint32_t *objectId, void *t, void *y, void *dy, uint64_t *rows);
+
+/* Best periods to a text file in the reference's formats: top_k = 1 gives
+ * "id, period, power" (outputPeriodsToFile, main.cu:961-976), top_k = 3
+ * gives "id, p1, p2, p3, pow1, pow2, pow3" (outputPeriodsToFileTopThree,
+ * main.cu:979-995); 6 significant digits.  periods/powers hold
+ * num_objects*top_k values of dtype. */
+int lsamd_write_periods(const char *path, const uint32_t *objectIds, uint32_t num_objects,
+                        uint32_t top_k, int dtype, const void *periods, const void *powers);
+
 /* Status of the last failed call on this thread (LSAMD_OK if none). */
 int lsamd_last_error_code(void);
 const char *lsamd_last_error(void);
@@ -237,9 +254,10 @@ int lsamd_device_count(void);
 void lsamd_set_timing(int enabled);
 double lsamd_last_sweep_kernel_ms(void);
 
-/* Tile shape of the sweep kernel: threads per workgroup (64..1024, a
- * multiple of 64) and frequencies per thread (1, 2, 4 or 8).  Zero restores
- * the built-in choice.  Returns LSAMD_ERR_INVALID for shapes not compiled. */
+/* Tile shape of the sweep kernel: threads per workgroup and frequencies per
+ * thread, among the shapes compiled for the form and variant (128..512
+ * threads, 1..16 frequencies).  Zero restores the built-in choice.  Returns
+ * LSAMD_ERR_INVALID for shapes not compiled. */
 int lsamd_set_tile(int form, int variant, int threads, int freqs_per_thread);
 /* Tile shape lsamd_sweep would use now (for reports). */
 void lsamd_get_tile(int form, int variant, int dtype, int *threads, int *freqs_per_thread);
