@@ -24,10 +24,14 @@ Reported (one JSON line on stdout):
                   from the committed rocprofv3 counter summary
   cpu_baseline    the reference's own CPU batch path (mode 4, OpenMP) timed on
                   the host on a bounded object sample of the same workload
-  generalized_batch, uneven_batch, refinement_batch   the generalized
-                  1000-object batch, the 100000-object top-3 batch (persistent,
-                  graph, autotuned) and the 4096-object grid search + Newton
-                  refinement + gradient workload
+  generalized_batch, uneven_batch, refinement_batch, single_object
+                  the generalized 1000-object batch, the 100000-object top-3
+                  batch (persistent, graph, autotuned), the 4096-object grid
+                  search + Newton refinement + gradient workload, and the
+                  3555-observation single object beside the reference's CPU
+                  MODE 5
+  end_to_end_pcie the 1000-object batch through LombScarglePy with host
+                  arrays (PCIe-inclusive; never the headline value)
 """
 from __future__ import annotations
 
@@ -305,6 +309,104 @@ def refinement_batch(torch, L, dev):
     return out
 
 
+def single_object(torch, L, dev, cpu_threads):
+    """One object, N = 3555 over 1800 nights, 1e5 angular freqs 1.005..150.796:
+    GPU plain FP64, split and split32 against the reference CPU single-object
+    path (MODE 5, OpenMP over frequencies, FP64), full run."""
+    import oracle
+    from gpu_lomb_scargle_amd import _native, device, synth
+    data = synth.single_object_3555()
+    nf = 100000
+    n = data["t"].size
+    terms = float(n) * nf
+    t, y, dy, off = device.numpy_to_device(None, data["t"], data["y"], None, np.float64, dev)
+    out = {"workload": "single object: N = 3555 over 1800 nights, 1e5 angular freqs 1.005..150.796, "
+                       "normalised standard power, periodogram + best period",
+           "terms_per_step": terms}
+    res = {}
+    for name, form in (("fp64_plain", _native.FORM_PLAIN), ("split", _native.FORM_SPLIT),
+                       ("split32", _native.FORM_SPLIT32)):
+        p = device.SweepPlan(t, y, None, off, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
+                             num_freqs=nf, variant=_native.STANDARD, form=form, top_k=1)
+        ms, kms = timed_plan(torch, L, p, 5, 2)
+        res[name] = p.results()
+        out[name] = {"value": terms / (ms / 1e3), "ms_per_step": round(ms, 4), "kernel_ms": round(kms, 4),
+                     "period": float(res[name]["period"][0, 0])}
+    ref = res["fp64_plain"]["pgram"]
+    for name in ("split", "split32"):
+        a = res[name]["pgram"]
+        m = ref >= 1e-3 * ref.max()
+        out[name]["max_rel_err_masked"] = float(((a - ref).abs() / ref.abs()).where(m, torch.zeros_like(a)).max())
+        out[name]["argmax_identical"] = bool(int(res[name]["index"][0, 0]) == int(res["fp64_plain"]["index"][0, 0]))
+    kind = "reference" if oracle.ref_available(False, np.float64, True, cpu_threads) else "port"
+    yy = data["y"].copy()
+    t0 = time.perf_counter()
+    if kind == "reference":
+        pg = oracle.ref_single(data["t"], yy, None, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, False, np.float64,
+                               threads=cpu_threads)
+    else:
+        pg = oracle.single(data["t"], yy, None, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, False, np.float64,
+                           nthreads=cpu_threads)
+    dt = time.perf_counter() - t0
+    pg = np.asarray(pg[0] if isinstance(pg, tuple) else pg).reshape(-1)
+    gpu = res["fp64_plain"]["pgram"].cpu().numpy().reshape(-1)
+    out["cpu_mode5"] = {"value": terms / dt, "seconds": round(dt, 3), "cores": cpu_threads, "kind": kind,
+                        "fp64_plain_max_abs_diff_over_pmax": float(np.abs(gpu - pg).max() / pg.max()),
+                        "argmax_identical": bool(int(np.argmax(pg)) == int(res["fp64_plain"]["index"][0, 0]))}
+    out["true_period"] = float(data["period"][0])
+    return out
+
+
+def end_to_end(torch, nf):
+    """The 1000-object batch through the drop-in entry point LombScarglePy
+    (libpylsnoerrordouble.so, MODE 1, LSAMD_FORM=split32): host arrays in,
+    the 8 GB periodogram copied back into pageable host memory.  PCIe-inclusive;
+    not the headline value."""
+    import ctypes as ct
+    from gpu_lomb_scargle_amd import _native, synth
+    data = synth.standard_batch_1000()
+    lib = _native.load(_native.DROPIN_LIBS[(False, "double")])
+    P = ct.POINTER
+    lib.LombScarglePy.argtypes = [P(ct.c_uint), P(ct.c_double), P(ct.c_double), P(ct.c_double), ct.c_uint,
+                                  ct.c_double, ct.c_double, ct.c_uint, ct.c_int, P(ct.c_double)]
+    lib.LombScarglePy.restype = None
+    o = np.ascontiguousarray(data["obj_id"], np.uint32)
+    t = np.ascontiguousarray(data["t"], np.float64)
+    y = np.ascontiguousarray(data["y"], np.float64)
+    dy = np.ones_like(t)
+    K = int(data["counts"].size)
+    pgram = np.empty(K * nf, np.float64)
+    pgram[::4096] = 0.0
+    old_form = os.environ.get("LSAMD_FORM")
+    os.environ["LSAMD_FORM"] = "split32"
+    os.environ["LSAMD_GPU_STATS"] = "0"
+    cp = lambda a, ty: a.ctypes.data_as(P(ty))
+    saved = os.dup(1)
+    times = []
+    try:
+        os.dup2(2, 1)  # the entry point prints progress lines on stdout, as the reference does
+        for _ in range(3):
+            t0 = time.perf_counter()
+            lib.LombScarglePy(cp(o, ct.c_uint), cp(t, ct.c_double), cp(y, ct.c_double), cp(dy, ct.c_double),
+                              o.size, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, 1, cp(pgram, ct.c_double))
+            times.append(time.perf_counter() - t0)
+    finally:
+        ct.CDLL(None).fflush(None)  # the C library's stdout buffer goes to stderr too
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+        if old_form is None:
+            del os.environ["LSAMD_FORM"]
+        else:
+            os.environ["LSAMD_FORM"] = old_form
+    dt = min(times[1:])
+    terms = float(t.size) * nf
+    return {"workload": "1000-object batch through LombScarglePy (MODE 1, double library, LSAMD_FORM=split32): "
+                        "host arrays in, 8 GB periodogram copied to pageable host memory",
+            "value": terms / dt, "unit": "terms/s", "seconds": round(dt, 3),
+            "host_copy_GBps": round(K * nf * 8 / dt / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -461,6 +563,10 @@ def main():
         log(f"uneven batch: {out['uneven_batch']}")
         out["refinement_batch"] = refinement_batch(torch, L, dev)
         log(f"refinement batch: {out['refinement_batch']}")
+        out["single_object"] = single_object(torch, L, dev, args.cpu_threads)
+        log(f"single object: {out['single_object']}")
+        out["end_to_end_pcie"] = end_to_end(torch, nf)
+        log(f"end to end: {out['end_to_end_pcie']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(data, nf, args.cpu_objects, args.cpu_threads)
         log(f"cpu baseline: {out['cpu_baseline']}")

@@ -38,7 +38,7 @@ EXPORTED = [
     "lsamd_last_sweep_kernel_ms", "lsamd_set_tile", "lsamd_get_tile", "lsamd_version",
     "lsamd_graph_create", "lsamd_graph_launch", "lsamd_graph_destroy", "lsamd_autotune",
     "lsamd_autotune_workspace_size", "lsamd_power_grad", "lsamd_refine_peaks", "lsamd_read_table",
-    "lsamd_write_periods",
+    "lsamd_write_periods", "lsamd_segment_workspace_size", "lsamd_segment",
 ]
 
 
@@ -123,6 +123,11 @@ def _declare(L):
     L.lsamd_read_table.restype = c_int
     L.lsamd_write_periods.argtypes = [c_char_p, c_void_p, c_uint32, c_uint32, c_int, c_void_p, c_void_p]
     L.lsamd_write_periods.restype = c_int
+    L.lsamd_segment_workspace_size.argtypes = [c_uint64, c_uint64]
+    L.lsamd_segment_workspace_size.restype = c_size_t
+    L.lsamd_segment.argtypes = [c_void_p, c_uint64, c_void_p, c_uint64, c_void_p, c_void_p, c_void_p,
+                                c_size_t, c_void_p]
+    L.lsamd_segment.restype = c_int
     return L
 
 

@@ -553,11 +553,15 @@ __global__ __launch_bounds__(T) void k_sweep_plain_gen(SweepArgs a) {
 // multiply and round for the seed phase (revolutions), one hardware sin/cos
 // pair, then M bins by FP32 rotation with the observation's precomputed step
 // (cos, sin of T bins); FP32 partial sums over 64 observations are folded
-// into FP64 accumulators.
+// into FP64 accumulators.  Every FP32 instruction of the inner loop is an
+// all-VGPR v_fmac_f32 / v_mul_f32 (the records carry -sin D so that no
+// negated operand forces the slower VOP3 encoding).
 // ---------------------------------------------------------------------------
 template <bool GEN, typename OUT, int T, int M>
 __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
-    constexpr int NS = GEN ? 7 : 5;
+    // standard: XC, XS, CC, CS (SS = N - CC); generalized: all seven sums
+    // (its centred SS - S^2 cancels, so SS is accumulated, not derived)
+    constexpr int NS = GEN ? 7 : 4;
     __shared__ double s_t[T];
     __shared__ float4 s_a[T];
     __shared__ float s_n[GEN ? T : 1];
@@ -620,8 +624,7 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
                             p[0][m] = fmaf(r.x, c, p[0][m]);
                             p[1][m] = fmaf(r.x, s, p[1][m]);
                             p[2][m] = fmaf(c, c, p[2][m]);
-                            p[3][m] = fmaf(s, s, p[3][m]);
-                            p[4][m] = fmaf(c, s, p[4][m]);
+                            p[3][m] = fmaf(c, s, p[3][m]);
                         }
                         if (m + 1 < M) {
                             const float cn = fmaf(c, r.z, s * nsd);
@@ -639,6 +642,7 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
         }
 
         const double scale = a.obj_scale[obj];
+        const double nobs = (double)(end - beg);
         double Wf = 1.0, Yb = 0.0, YY = 1.0;
         if constexpr (GEN) {
             Wf = a.obj_gen[kGen * obj + 0];
@@ -661,7 +665,7 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
                 P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
             } else {
                 const double XC = acc[0][m], XS = acc[1][m];
-                const double CC = acc[2][m], SS = acc[3][m], CS = acc[4][m];
+                const double CC = acc[2][m], CS = acc[3][m], SS = nobs - CC;
                 const double D = CC * SS - CS * CS;
                 P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
             }
@@ -893,6 +897,7 @@ const TileEntry kTiles[] = {
     LS_SPLIT32(1, float, LSAMD_F32, 256, 3), LS_SPLIT32(1, double, LSAMD_F64, 256, 3),
     LS_SPLIT(0, float, LSAMD_F32, 256, 12), LS_SPLIT(0, double, LSAMD_F64, 256, 12),
     LS_SPLIT32(0, float, LSAMD_F32, 128, 6), LS_SPLIT32(0, double, LSAMD_F64, 128, 6),
+    LS_SPLIT32(0, float, LSAMD_F32, 128, 8), LS_SPLIT32(0, double, LSAMD_F64, 128, 8),
     LS_SPLIT32(1, float, LSAMD_F32, 128, 3), LS_SPLIT32(1, double, LSAMD_F64, 128, 3),
     // plain form
     LS_PLAIN_STD(float, LSAMD_F32, 256, 1), LS_PLAIN_STD(double, LSAMD_F64, 256, 1),

@@ -161,6 +161,32 @@ def sweep(t, y, dy, offsets, **kw):
     return plan.results()
 
 
+def segment(obj_id, max_objects=None, stream=None):
+    """Device segmentation (lsamd_segment) of a CUDA uint32/int32 tensor of
+    object ids: returns (offsets int64 [K+1], order int32 [K]), objects being
+    maximal runs of equal consecutive ids and the order longest class first.
+    One synchronisation (to read K)."""
+    if not obj_id.is_cuda:
+        raise RuntimeError("segment needs a CUDA (HIP) tensor")
+    lib = _native.core()
+    ids = obj_id.to(torch.int32).contiguous()
+    n = ids.numel()
+    cap = n if max_objects is None else int(max_objects)
+    dev = ids.device
+    off = torch.empty(cap + 1, dtype=torch.int64, device=dev)
+    order = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+    k = torch.empty(1, dtype=torch.int32, device=dev)
+    ws = torch.empty(int(lib.lsamd_segment_workspace_size(n, cap)), dtype=torch.uint8, device=dev)
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    _native.check(lib, lib.lsamd_segment(c_void_p(ids.data_ptr()), n, c_void_p(off.data_ptr()), cap,
+                                         c_void_p(order.data_ptr()), c_void_p(k.data_ptr()),
+                                         c_void_p(ws.data_ptr()), ws.numel(), c_void_p(s.cuda_stream)))
+    K = int(k.item()) & 0xFFFFFFFF
+    if K > cap:
+        raise ValueError(f"{K} objects exceed max_objects={cap}")
+    return off[:K + 1], order[:K]
+
+
 def numpy_to_device(obj_id, t, y, dy, dtype=np.float64, device="cuda"):
     """Host observation arrays -> (t, y, dy, offsets) CUDA tensors, objects
     as runs of obj_id (None = one object)."""

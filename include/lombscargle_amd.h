@@ -223,6 +223,21 @@ uint32_t lsamd_count_objects(const uint32_t *objectId, uint64_t n);
 uint32_t lsamd_object_offsets(const uint32_t *objectId, uint64_t n, uint64_t *offsets,
                               uint32_t *ids);
 
+/* Device segmentation (replaces computeObjectRanges, source/main.cu:468-510,
+ * for ids already in device memory): offsets[0..K] (device, room for
+ * max_objects + 1) of the maximal runs of equal consecutive ids in
+ * objectId[n] (device), K in *num_objects (device uint32), and, when order
+ * (device, room for max_objects) is not NULL, a processing order for
+ * lsamd_sweep: objects by decreasing floor(log2 N_k), ties in index order.
+ * If K > max_objects, *num_objects still receives K (saturated at 2^32-1)
+ * and nothing else is written.  Stream-ordered; does not allocate; no
+ * memset.  The workspace is device memory of
+ * lsamd_segment_workspace_size(n, max_objects) bytes. */
+size_t lsamd_segment_workspace_size(uint64_t n, uint64_t max_objects);
+int lsamd_segment(const uint32_t *objectId, uint64_t n, uint64_t *offsets, uint64_t max_objects,
+                  uint32_t *order, uint32_t *num_objects, void *workspace, size_t workspace_bytes,
+                  void *stream);
+
 /* Observation table in the reference's input format (importObjXYData,
  * source/main.cu:1781-1843): numbers separated by commas and/or whitespace,
  * rows of objectId, t, y (and dy when with_dy), each value parsed as dtype

@@ -434,3 +434,32 @@ def test_autotune_picks_a_compiled_tile():
         assert_close(plan.results()["pgram"][:20].cpu().numpy(), ref[0], 1e-5, 1e-3, 2e-6)
     finally:
         L.lsamd_set_tile(_native.FORM_SPLIT32, _native.STANDARD, 0, 0)
+
+
+def test_device_segmentation_matches_host_runs():
+    """lsamd_segment against the host run rule (computeObjectRanges) and a
+    stable longest-class-first order; the overflow case writes nothing."""
+    import torch
+    from gpu_lomb_scargle_amd import device
+    rng = np.random.default_rng(17)
+    for n_obj, lo, hi in ((1, 1, 5), (37, 1, 9), (3000, 1, 700), (5, 3000, 9000)):
+        lens = rng.integers(lo, hi + 1, n_obj)
+        vals = rng.integers(0, 6, n_obj)  # repeats of an id later in the stream start new objects
+        for k in range(1, n_obj):
+            if vals[k] == vals[k - 1]:
+                vals[k] = vals[k - 1] + 1
+        ids = np.repeat(vals, lens).astype(np.uint32)
+        L = _native.core()
+        K = int(L.lsamd_count_objects(_native.ptr(ids), ids.size))
+        off = np.zeros(K + 1, np.uint64)
+        L.lsamd_object_offsets(_native.ptr(ids), ids.size, _native.ptr(off), None)
+        d_off, d_order = device.segment(torch.tensor(ids.astype(np.int32), device="cuda"))
+        assert d_off.cpu().numpy().tolist() == off.astype(np.int64).tolist()
+        n_k = np.diff(off.astype(np.int64))
+        cls = np.floor(np.log2(n_k)).astype(int)
+        want = np.argsort(-cls, kind="stable")
+        assert d_order.cpu().numpy().tolist() == want.tolist()
+    # overflow: K > max_objects leaves the outputs alone and reports K
+    ids = np.repeat(np.arange(10), 3).astype(np.int32)
+    with pytest.raises(ValueError):
+        device.segment(torch.tensor(ids, device="cuda"), max_objects=4)
