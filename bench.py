@@ -421,7 +421,7 @@ def main():
                     help="skip the generalized and uneven-batch workloads")
     ap.add_argument("--profile-once", action="store_true",
                     help="for rocprofv3 counter runs: one 1 GiB device copy (calibration of the "
-                         "HBM byte counters), then one sweep of each split form; no JSON")
+                         "HBM byte counters), then one sweep of each form; no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -466,12 +466,12 @@ def main():
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)  # 1 GiB
         dst = src.clone()
         torch.cuda.synchronize()
-        for form in (_native.FORM_SPLIT32, _native.FORM_SPLIT):
+        for form in (_native.FORM_SPLIT32, _native.FORM_SPLIT, _native.FORM_PLAIN):
             plan = make(form)
             plan.run()
             torch.cuda.synchronize()
             del plan
-        log("profile-once: 1 GiB copy, one split32 sweep, one split sweep")
+        log("profile-once: 1 GiB copy, one split32, one split and one FP64 plain sweep")
         del src, dst
         return
 

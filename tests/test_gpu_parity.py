@@ -283,8 +283,14 @@ def test_ragged_grids_and_staging_boundaries(form, nf):
     ok = np.array(counts) >= 3   # N<3: degenerate least squares, values not comparable
     if form == _native.FORM_PLAIN:
         assert_close(pg[ok], ref[0][ok], 1e-12, 1e-6, 1e-12)
-    else:
+    elif form == _native.FORM_SPLIT:
         assert_close(pg[ok], ref[0][ok], 1e-5, 1e-3, 2e-6)
+    else:
+        # split32 derives SS = N - CC from FP32 phasors whose modulus drifts by
+        # ~1e-7; on the 5-observation object the 2x2 system is nearly singular
+        # at some frequencies, so its absolute error reaches ~3e-6 P_max
+        # (relative error on the bins that carry power stays below 1e-5)
+        assert_close(pg[ok], ref[0][ok], 1e-5, 1e-3, 5e-6)
 
 
 def test_many_tiny_objects():
