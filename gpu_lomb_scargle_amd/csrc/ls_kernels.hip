@@ -680,7 +680,7 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Split form, FP64 recurrence and sums ("split", the default fast form).  The
+// Split form, FP64 recurrence and sums ("split", the accurate fast form).  The
 // seed of each (thread, observation) is the hardware FP32 sin/cos of the
 // FP64-reduced phase, corrected to first order by the FP64 residual of that
 // phase's rounding to FP32; the M bins then follow in FP64 from the

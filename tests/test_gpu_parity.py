@@ -110,6 +110,22 @@ def test_dropin_gpu_batch_mode1(dtype, error):
     assert argmax_agrees(pg, ref, 10 * TOL[dtype][0]) == 0
 
 
+@pytest.mark.parametrize("form", ["split", "split32"])
+@pytest.mark.parametrize("error", [False, True])
+def test_dropin_wrapper_fast_forms(form, error):
+    """The reference-compatible wrapper with form= (LSAMD_FORM for the
+    drop-in library): FP64 library, SDSS data, against the FP64 oracle at the
+    split forms' tolerances (split32 on these aliased data: 1e-4)."""
+    o, t, y, dy = load_data("SDSS_stripe82_band_z.txt")
+    ids, per, pg = ls.lombscargle(o, t, y.copy(), 0.1, 10.0, error, "GPU", dy if error else None,
+                                  3000, "double", False, form=form)
+    ref, rper = oracle.batch(o, t, y, dy if error else None, TWO_PI * 0.1, TWO_PI * 10.0, 3000, error, F64,
+                             nthreads=8)[:2]
+    assert "LSAMD_FORM" not in os.environ  # the wrapper restores the environment
+    assert_close(pg, ref, 1e-5 if form == "split" else 1e-4, 1e-3, 2e-6 if form == "split" else 2e-5)
+    assert argmax_agrees(pg, ref, 1e-4) == 0
+
+
 @pytest.mark.parametrize("dtype", ["double", "float"])
 @pytest.mark.parametrize("error", [False, True])
 @pytest.mark.parametrize("fname", ["8205_normalized_with_error.txt", "243_normalized_ztf_filter2.txt"])
