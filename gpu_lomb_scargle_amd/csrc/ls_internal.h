@@ -37,7 +37,7 @@ struct Workspace {
     size_t rec_y = 0;       // DTYPE[n]   plain: centred y
     size_t rec_w = 0;       // DTYPE[n]   plain generalized: (1/dy)^2 in DTYPE
     size_t rec_t = 0;       // double[n]  split: t
-    size_t rec_a = 0;       // float4[n]  split: {y or w*y, w or -sin step, cos step, sin step}
+    size_t rec_a = 0;       // float4[n]  split: {y or sqrt(w)(y-ybar), -sin step or sqrt(w), cos, sin step}
     size_t rec_n = 0;       // float[n]   split generalized: -sin step
     size_t rec_r = 0;       // double2[n] split: {cos step, sin step}
     // per (object, chunk) peak partials

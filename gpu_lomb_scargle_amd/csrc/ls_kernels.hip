@@ -155,7 +155,7 @@ struct RecArgs {
     const double *obj_shift, *obj_gen;
     void *rec_y, *rec_w;       // plain
     double *rec_t;             // split
-    float4 *rec_a;             // split: {y or w*y, w or -sin step, cos step, sin step} (FP32)
+    float4 *rec_a;             // split: {y or sqrt(w)(y-ybar), -sin step or sqrt(w), cos step, sin step} (FP32)
     float *rec_n;              // split generalized: -sin step (FP32)
     double2 *rec_r;            // split: {cos step, sin step} (FP64)
     double step_rev;           // split: T * df (revolutions/day per T bins)
@@ -228,7 +228,10 @@ __global__ void k_obj_stats(StatsArgs a) {
     }
     if (a.split_gen) {
         // weights 1/dy^2 normalised to unit sum; y centred on its weighted
-        // mean (the generalized power does not depend on the shift)
+        // mean (the generalized power does not depend on the shift).  The
+        // sweep carries sqrt(w) in each observation's phasor, so the weights
+        // it applies are exactly qf^2 with qf = (float)sqrt(w): the same
+        // values are summed here (qf^2 is exact in FP64).
         double W = 0.0, WY = 0.0;
         for (uint64_t i = b; i < e; ++i) {
             const double wi = 1.0 / ((double)dy[i] * (double)dy[i]);
@@ -238,12 +241,13 @@ __global__ void k_obj_stats(StatsArgs a) {
         const double ybar = WY / W;
         double Wf = 0.0, Ys = 0.0, YYs = 0.0;
         for (uint64_t i = b; i < e; ++i) {
-            const double wi = 1.0 / ((double)dy[i] * (double)dy[i]) / W;
+            const double qi = sqrt(1.0 / ((double)dy[i] * (double)dy[i]) / W);
             const double yc = (double)y[i] - ybar;
-            const float wn = (float)wi, wy = (float)(wi * yc);
-            Wf += (double)wn;
-            Ys += (double)wy;
-            YYs += (double)wn * yc * yc;
+            const double q = (double)(float)qi;
+            const float bq = (float)(qi * yc);
+            Wf += q * q;
+            Ys += (double)bq * q;
+            YYs += q * q * yc * yc;
         }
         const double Yb = Ys / Wf;
         a.obj_gen[kGen * k + 0] = Wf;
@@ -299,9 +303,10 @@ __global__ void k_records_split(RecArgs a) {
         // all-VGPR v_fmac_f32 issues at twice the rate of a negated VOP3 form).
         float4 r;
         if (GEN) {
-            const double wi = 1.0 / ((double)dy[i] * (double)dy[i]) / W;
-            r.x = (float)(wi * ((double)y[i] - ybar));
-            r.y = (float)wi;
+            // {sqrt(w) (y - ybar), sqrt(w)}: the sweep scales the phasor by sqrt(w)
+            const double qi = sqrt(1.0 / ((double)dy[i] * (double)dy[i]) / W);
+            r.x = (float)(qi * ((double)y[i] - ybar));
+            r.y = (float)qi;
             a.rec_n[i] = -(float)sv;
         } else {
             r.x = (float)y[i];
@@ -609,17 +614,23 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
                     const float phf = (float)ph;
                     float c = __builtin_amdgcn_cosf(phf);
                     float s = __builtin_amdgcn_sinf(phf);
+                    if constexpr (GEN) {
+                        // u = sqrt(w) z: the rotation is linear, and every
+                        // weighted sum becomes one FMA (w c = sqrt(w) u_c,
+                        // w c^2 = u_c^2, w y c = (sqrt(w) y) u_c)
+                        c *= r.y;
+                        s *= r.y;
+                    }
     #pragma unroll
                     for (int m = 0; m < M; ++m) {
                         if constexpr (GEN) {
-                            const float wc = r.y * c, ws = r.y * s;
-                            p[0][m] += wc;
-                            p[1][m] += ws;
+                            p[0][m] = fmaf(r.y, c, p[0][m]);
+                            p[1][m] = fmaf(r.y, s, p[1][m]);
                             p[2][m] = fmaf(r.x, c, p[2][m]);
                             p[3][m] = fmaf(r.x, s, p[3][m]);
-                            p[4][m] = fmaf(wc, c, p[4][m]);
-                            p[5][m] = fmaf(ws, s, p[5][m]);
-                            p[6][m] = fmaf(wc, s, p[6][m]);
+                            p[4][m] = fmaf(c, c, p[4][m]);
+                            p[5][m] = fmaf(s, s, p[5][m]);
+                            p[6][m] = fmaf(c, s, p[6][m]);
                         } else {
                             p[0][m] = fmaf(r.x, c, p[0][m]);
                             p[1][m] = fmaf(r.x, s, p[1][m]);
@@ -739,20 +750,20 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
                 double s = fma(res, (double)cf, (double)sf);
                 // one Newton step towards |z| = 1: the hardware sin/cos leave
                 // |z|^2 - 1 ~ 1e-7, which SS = W - CC would otherwise carry
-                const double nf = fma(-0.5, fma(c, c, s * s), 1.5);
+                double nf = fma(-0.5, fma(c, c, s * s), 1.5);
+                const double yv = (double)yw.x;
+                const double qv = (double)yw.y;  // generalized: sqrt(w)
+                if constexpr (GEN) nf *= qv;     // u = sqrt(w) z, |u|^2 = w
                 c *= nf;
                 s *= nf;
-                const double yv = (double)yw.x;
-                const double wv = (double)yw.y;
                 auto accumulate = [&](int m, double cv, double sv) {
                     if constexpr (GEN) {
-                        const double wc = wv * cv;
-                        acc[0][m] += wc;
-                        acc[1][m] = fma(wv, sv, acc[1][m]);
+                        acc[0][m] = fma(qv, cv, acc[0][m]);
+                        acc[1][m] = fma(qv, sv, acc[1][m]);
                         acc[2][m] = fma(yv, cv, acc[2][m]);
                         acc[3][m] = fma(yv, sv, acc[3][m]);
-                        acc[4][m] = fma(wc, cv, acc[4][m]);
-                        acc[5][m] = fma(wc, sv, acc[5][m]);
+                        acc[4][m] = fma(cv, cv, acc[4][m]);
+                        acc[5][m] = fma(cv, sv, acc[5][m]);
                     } else {
                         acc[0][m] = fma(yv, cv, acc[0][m]);
                         acc[1][m] = fma(yv, sv, acc[1][m]);
@@ -919,11 +930,11 @@ int g_tile_M[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
 
 // Built-in shapes: the fastest of the tile sweep on the 1000-object x 1e6
 // batches (MI355X): split 256x16 (standard) / 256x8 (generalized); split32
-// 256x6 (standard) / 128x4 (generalized).  lsamd_autotune re-picks per
+// 256x6 (standard) / 256x4 (generalized).  lsamd_autotune re-picks per
 // workload.
 Tile default_tile(int form, int variant) {
     if (form == LSAMD_FORM_SPLIT) return variant ? Tile{256, 8} : Tile{256, 16};
-    if (form == LSAMD_FORM_SPLIT32) return variant ? Tile{128, 4} : Tile{256, 6};
+    if (form == LSAMD_FORM_SPLIT32) return variant ? Tile{256, 4} : Tile{256, 6};
     return Tile{256, 1};
 }
 
