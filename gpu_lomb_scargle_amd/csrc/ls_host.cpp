@@ -95,11 +95,13 @@ struct Ctx {
     std::mutex mu;
     bool ready = false;
     int device = -1;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // uploads and sweeps
+    hipStream_t copy_stream = nullptr;  // periodogram copy-back
     DevBuf t, y, dy, off, order, ws, pgram, pidx, ppow, pper;
     void *pinned[2] = {nullptr, nullptr};
     size_t pinned_cap = 0;
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};     // staging chunks (copy stream)
+    hipEvent_t slice[2] = {nullptr, nullptr};  // sweep of a slice done (sweep stream)
 };
 
 Ctx &ctx() {
@@ -122,8 +124,11 @@ int ctx_init(Ctx &c) {
         return set_error(LSAMD_ERR_NODEVICE, "no HIP device visible");
     HIP_OK(hipGetDevice(&c.device));
     HIP_OK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    HIP_OK(hipEventCreateWithFlags(&c.ev[0], hipEventDisableTiming));
-    HIP_OK(hipEventCreateWithFlags(&c.ev[1], hipEventDisableTiming));
+    HIP_OK(hipStreamCreateWithFlags(&c.copy_stream, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+        HIP_OK(hipEventCreateWithFlags(&c.ev[i], hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&c.slice[i], hipEventDisableTiming));
+    }
     c.ready = true;
     return LSAMD_OK;
 }
@@ -143,13 +148,14 @@ int ensure(DevBuf &b, size_t bytes) {
 
 constexpr size_t kPinnedChunk = size_t(64) << 20;  // 64 MiB per staging buffer
 
-// Device -> pageable host copy through two pinned staging buffers, so the
-// DMA of one chunk overlaps the host copy of the previous one.
+// Device -> pageable host copy on the copy stream through two pinned staging
+// buffers, so the DMA of one chunk overlaps the host copy of the previous one.
+// Returns when the data is in dst.
 int copy_to_host(Ctx &c, void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return LSAMD_OK;
     if (bytes <= (size_t(8) << 20)) {
-        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
-        HIP_OK(hipStreamSynchronize(c.stream));
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.copy_stream));
+        HIP_OK(hipStreamSynchronize(c.copy_stream));
         return LSAMD_OK;
     }
     if (c.pinned_cap < kPinnedChunk) {
@@ -164,13 +170,13 @@ int copy_to_host(Ctx &c, void *dst, const void *src, size_t bytes) {
     auto chunk_len = [&](size_t i) { return std::min(kPinnedChunk, bytes - i * kPinnedChunk); };
     const char *s = static_cast<const char *>(src);
     char *d = static_cast<char *>(dst);
-    HIP_OK(hipMemcpyAsync(c.pinned[0], s, chunk_len(0), hipMemcpyDeviceToHost, c.stream));
-    HIP_OK(hipEventRecord(c.ev[0], c.stream));
+    HIP_OK(hipMemcpyAsync(c.pinned[0], s, chunk_len(0), hipMemcpyDeviceToHost, c.copy_stream));
+    HIP_OK(hipEventRecord(c.ev[0], c.copy_stream));
     for (size_t i = 1; i <= nch; ++i) {
         if (i < nch) {
             HIP_OK(hipMemcpyAsync(c.pinned[i & 1], s + i * kPinnedChunk, chunk_len(i),
-                                  hipMemcpyDeviceToHost, c.stream));
-            HIP_OK(hipEventRecord(c.ev[i & 1], c.stream));
+                                  hipMemcpyDeviceToHost, c.copy_stream));
+            HIP_OK(hipEventRecord(c.ev[i & 1], c.copy_stream));
         }
         HIP_OK(hipEventSynchronize(c.ev[(i - 1) & 1]));
         const size_t len = chunk_len(i - 1);
@@ -299,13 +305,29 @@ int lsamd_batch_host(int variant, int dtype, int form, int normalize, int precen
     HIP_OK(hipMemcpyAsync(c.off.p, off.data(), ((size_t)K + 1) * 8, hipMemcpyHostToDevice,
                           c.stream));
 
-    // longest objects first inside every batch (batch-local indices)
-    std::vector<uint32_t> order(K);
+    // Each memory-budget batch is swept in slices of objects: while the
+    // periodogram of slice s is copied back on the copy stream, slice s+1 is
+    // swept, so the end-to-end time approaches the copy time alone.  Slices
+    // hold ~2 GiB of periodogram (at least 1 object; LSAMD_SLICE_BYTES
+    // overrides): each slice's copy restarts the two-buffer staging pipeline,
+    // so slices much smaller than that lose copy bandwidth.
+    const size_t kSliceBytes = (size_t)env_double("LSAMD_SLICE_BYTES", (double)(size_t(2) << 30));
+    const uint32_t objs_per_slice =
+        per_obj > 0 ? (uint32_t)std::max<size_t>(1, std::min<size_t>(objs_per_batch, kSliceBytes / per_obj))
+                    : objs_per_batch;
+    std::vector<uint32_t> slice_start;
     for (uint32_t b0 = 0; b0 < K; b0 += objs_per_batch) {
         const uint32_t b1 = std::min(K, b0 + objs_per_batch);
-        for (uint32_t k = b0; k < b1; ++k) order[k] = k - b0;
-        std::stable_sort(order.begin() + b0, order.begin() + b1, [&](uint32_t a, uint32_t b) {
-            return off[b0 + a + 1] - off[b0 + a] > off[b0 + b + 1] - off[b0 + b];
+        for (uint32_t s0 = b0; s0 < b1; s0 += objs_per_slice) slice_start.push_back(s0);
+    }
+    // longest objects first inside every slice (slice-local indices)
+    std::vector<uint32_t> order(K);
+    for (size_t i = 0; i < slice_start.size(); ++i) {
+        const uint32_t s0 = slice_start[i];
+        const uint32_t s1 = i + 1 < slice_start.size() ? std::min(slice_start[i + 1], K) : K;
+        for (uint32_t k = s0; k < s1; ++k) order[k] = k - s0;
+        std::stable_sort(order.begin() + s0, order.begin() + s1, [&](uint32_t a, uint32_t b) {
+            return off[s0 + a + 1] - off[s0 + a] > off[s0 + b + 1] - off[s0 + b];
         });
     }
     HIP_OK(hipMemcpyAsync(c.order.p, order.data(), (size_t)K * 4, hipMemcpyHostToDevice,
@@ -323,39 +345,80 @@ int lsamd_batch_host(int variant, int dtype, int form, int normalize, int precen
         if (rc) return rc;
     }
 
-    for (uint32_t b0 = 0; b0 < K; b0 += objs_per_batch) {
-        const uint32_t kb = std::min(K - b0, objs_per_batch);
+    // slice i covers objects [s0, s1) and, inside its batch, periodogram rows
+    // from (s0 - batch start); a slice never spans two batches
+    auto slice_end = [&](size_t i) {
+        const uint32_t s0 = slice_start[i];
+        const uint32_t batch_end = std::min(K, (s0 / objs_per_batch) * objs_per_batch + objs_per_batch);
+        return std::min(batch_end, s0 + objs_per_slice);
+    };
+    auto batch_row = [&](uint32_t k) { return (size_t)(k % objs_per_batch); };
+    auto sweep_slice = [&](size_t i) -> int {
+        const uint32_t s0 = slice_start[i], s1 = slice_end(i);
         lsamd_sweep_args a{};
         a.variant = variant;
         a.dtype = dtype;
         a.form = form;
         a.normalize = normalize;
         a.precenter = 0;  // already done on the host, in place
-        a.num_objects = kb;
+        a.num_objects = s1 - s0;
         a.num_freqs = nf;
         a.top_k = tk;
         a.num_obs = n;
         a.min_freq = fmin;
         a.max_freq = fmax;
-        a.offsets = static_cast<const uint64_t *>(c.off.p) + b0;
-        a.order = static_cast<const uint32_t *>(c.order.p) + b0;
+        a.offsets = static_cast<const uint64_t *>(c.off.p) + s0;
+        a.order = static_cast<const uint32_t *>(c.order.p) + s0;
         a.t = c.t.p;
         a.y = c.y.p;
         a.dy = variant == LSAMD_GENERALIZED ? c.dy.p : nullptr;
-        a.pgram = pgram ? c.pgram.p : nullptr;
+        a.pgram = pgram ? static_cast<char *>(c.pgram.p) + batch_row(s0) * per_obj : nullptr;
         if (tk) {
-            a.peak_index = static_cast<uint32_t *>(c.pidx.p) + (size_t)b0 * tk;
-            a.peak_power = static_cast<char *>(c.ppow.p) + (size_t)b0 * tk * rsz;
-            a.peak_period = static_cast<char *>(c.pper.p) + (size_t)b0 * tk * rsz;
+            a.peak_index = static_cast<uint32_t *>(c.pidx.p) + (size_t)s0 * tk;
+            a.peak_power = static_cast<char *>(c.ppow.p) + (size_t)s0 * tk * rsz;
+            a.peak_period = static_cast<char *>(c.pper.p) + (size_t)s0 * tk * rsz;
         }
         const size_t wsb = lsamd_sweep_workspace_size(&a);
-        rc = ensure(c.ws, wsb);
-        if (rc) return rc;
-        rc = lsamd_sweep(&a, c.ws.p, c.ws.cap, c.stream);
-        if (rc) return rc;
+        int r = LSAMD_OK;
+        if (c.ws.cap < wsb) {
+            // growing the workspace frees it: wait for the sweeps using it
+            HIP_OK(hipStreamSynchronize(c.stream));
+            r = ensure(c.ws, wsb);
+            if (r) return r;
+        }
+        r = lsamd_sweep(&a, c.ws.p, c.ws.cap, c.stream);
+        if (r) return r;
+        HIP_OK(hipEventRecord(c.slice[i & 1], c.stream));
+        return LSAMD_OK;
+    };
+    auto copy_slice = [&](size_t i) -> int {
+        const uint32_t s0 = slice_start[i], s1 = slice_end(i);
+        HIP_OK(hipStreamWaitEvent(c.copy_stream, c.slice[i & 1], 0));
+        return copy_to_host(c, static_cast<char *>(pgram) + (size_t)s0 * per_obj,
+                            static_cast<char *>(c.pgram.p) + batch_row(s0) * per_obj,
+                            (size_t)(s1 - s0) * per_obj);
+    };
+    const size_t ns = slice_start.size();
+    rc = sweep_slice(0);
+    if (rc) return rc;
+    for (size_t i = 1; i <= ns; ++i) {
+        // Within a batch, slice i writes other device rows than slice i-1, so
+        // it is swept while slice i-1 is copied back.  The first slice of a
+        // new batch reuses rows from 0, which the still-pending last slice of
+        // the previous batch may occupy: that slice is copied back first
+        // (copy_slice returns once the data is in host memory).
+        const bool next = i < ns;
+        const bool new_batch = next && slice_start[i] % objs_per_batch == 0;
+        if (next && !new_batch) {
+            rc = sweep_slice(i);
+            if (rc) return rc;
+        }
         if (pgram) {
-            rc = copy_to_host(c, static_cast<char *>(pgram) + (size_t)b0 * per_obj, c.pgram.p,
-                              (size_t)kb * per_obj);
+            rc = copy_slice(i - 1);
+            if (rc) return rc;
+        }
+        if (new_batch) {
+            rc = sweep_slice(i);
             if (rc) return rc;
         }
     }

@@ -330,6 +330,25 @@ def test_memory_budget_batching_is_exact():
     finally:
         del os.environ["LSAMD_PGRAM_BUDGET_BYTES"]
     assert np.array_equal(full["pgram"], part["pgram"])
+
+
+def test_sliced_copy_back_is_exact():
+    """Batches of 7 objects swept in slices of 3 (the copy-back of one slice
+    overlaps the sweep of the next): bit-identical to one batch."""
+    b = synth.lognormal_batch(23, 100, 0.5, 20, 400, 1800, False, 19)
+    full = _native.batch_host(b["obj_id"], b["t"], b["y"], None, 1.0, 60.0, 3000, _native.STANDARD, F64,
+                              _native.FORM_SPLIT32, top_k=3)
+    os.environ["LSAMD_PGRAM_BUDGET_BYTES"] = str(7 * 3000 * 8)
+    os.environ["LSAMD_SLICE_BYTES"] = str(3 * 3000 * 8)
+    try:
+        part = _native.batch_host(b["obj_id"], b["t"], b["y"], None, 1.0, 60.0, 3000, _native.STANDARD, F64,
+                                  _native.FORM_SPLIT32, top_k=3)
+    finally:
+        del os.environ["LSAMD_PGRAM_BUDGET_BYTES"]
+        del os.environ["LSAMD_SLICE_BYTES"]
+    assert np.array_equal(full["pgram"], part["pgram"])
+    for key in ("index", "power", "period"):
+        assert np.array_equal(full[key], part[key])
     assert np.array_equal(full["index"], part["index"])
 
 
