@@ -907,6 +907,7 @@ const TileEntry kTiles[] = {
     LS_SPLIT32(0, float, LSAMD_F32, 256, 6), LS_SPLIT32(0, double, LSAMD_F64, 256, 6),
     LS_SPLIT32(1, float, LSAMD_F32, 256, 3), LS_SPLIT32(1, double, LSAMD_F64, 256, 3),
     LS_SPLIT(0, float, LSAMD_F32, 256, 12), LS_SPLIT(0, double, LSAMD_F64, 256, 12),
+    LS_SPLIT(0, float, LSAMD_F32, 256, 14), LS_SPLIT(0, double, LSAMD_F64, 256, 14),
     LS_SPLIT32(0, float, LSAMD_F32, 128, 6), LS_SPLIT32(0, double, LSAMD_F64, 128, 6),
     LS_SPLIT32(0, float, LSAMD_F32, 128, 8), LS_SPLIT32(0, double, LSAMD_F64, 128, 8),
     LS_SPLIT32(1, float, LSAMD_F32, 128, 3), LS_SPLIT32(1, double, LSAMD_F64, 128, 3),
