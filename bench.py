@@ -24,8 +24,9 @@ Reported (one JSON line on stdout):
                   from the committed rocprofv3 counter summary
   cpu_baseline    the reference's own CPU batch path (mode 4, OpenMP) timed on
                   the host on a bounded object sample of the same workload
-  generalized_batch, uneven_batch, refinement_batch, single_object
-                  the generalized 1000-object batch, the 100000-object top-3
+  fp32_batch, generalized_batch, uneven_batch, refinement_batch, single_object
+                  the same batch with FP32 inputs and periodogram, the
+                  generalized 1000-object batch, the 100000-object top-3
                   batch (persistent, graph, autotuned), the 4096-object grid
                   search + Newton refinement + gradient workload, and the
                   3555-observation single object beside the reference's CPU
@@ -141,6 +142,37 @@ def timed_plan(torch, L, plan, steps, warmup, stream=None):
     t1 = time.perf_counter()
     L.lsamd_set_timing(0)
     return (t1 - t0) * 1e3 / steps, float(np.mean(kms))
+
+
+def fp32_batch(torch, L, dev, data, nf, steps):
+    """The same 1000-object batch with FP32 inputs and FP32 periodogram (the
+    reference's float build): plain FP32 (the reference's float arithmetic,
+    sin/cos/atan2 of float arguments correctly rounded) and split32, the
+    latter checked against FP64 on the float-rounded inputs."""
+    from gpu_lomb_scargle_amd import _native, device, synth
+    K = int(data["counts"].size)
+    terms = float(data["counts"].sum()) * nf
+    out = {"workload": "config-1 batch with FP32 inputs and FP32 periodogram", "terms_per_step": terms}
+    t32, y32, _, off = device.numpy_to_device(data["obj_id"], data["t"], data["y"], None, np.float32, dev)
+    mk = lambda t, y, form: device.SweepPlan(t, y, None, off, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
+                                             num_freqs=nf, variant=_native.STANDARD, form=form, top_k=1)
+    p = mk(t32, y32, _native.FORM_PLAIN)
+    ms, kms = timed_plan(torch, L, p, 1, 1)
+    out["fp32_plain"] = {"value": terms / (ms / 1e3), "ms_per_step": round(ms, 3), "kernel_ms": round(kms, 3)}
+    del p
+    p = mk(t32, y32, _native.FORM_SPLIT32)
+    ms, kms = timed_plan(torch, L, p, steps, 1)
+    ref = mk(t32.double(), y32.double(), _native.FORM_PLAIN)
+    ref.run()
+    torch.cuda.synchronize()
+    res = p.results()
+    res = {k: (v.double() if v.is_floating_point() else v) for k, v in res.items()}
+    out["split32"] = {"value": terms / (ms / 1e3), "ms_per_step": round(ms, 3), "kernel_ms": round(kms, 3),
+                      "accuracy": full_accuracy(torch, res, ref.results(), K),
+                      "note": "against FP64 on the float-rounded inputs"}
+    del p, ref, res
+    torch.cuda.empty_cache()
+    return out
 
 
 def generalized_batch(torch, L, dev, steps):
@@ -557,6 +589,8 @@ def main():
         runs.clear()
         del plain, pp
         torch.cuda.empty_cache()
+        out["fp32_batch"] = fp32_batch(torch, L, dev, data, nf, max(2, args.steps // 2))
+        log(f"fp32 batch: {out['fp32_batch']}")
         out["generalized_batch"] = generalized_batch(torch, L, dev, max(2, args.steps // 2))
         log(f"generalized batch: {out['generalized_batch']}")
         out["uneven_batch"] = uneven_batch(torch, L, dev, max(2, args.steps // 2))
