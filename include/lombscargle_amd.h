@@ -267,14 +267,18 @@ int lsamd_device_count(void);
 
 /* When enabled, lsamd_sweep brackets its dominant kernel with HIP events on
  * its stream; lsamd_last_sweep_kernel_ms() returns that duration after the
- * stream has been synchronised (negative if unavailable). */
+ * stream has been synchronised (negative if unavailable).  Process-wide
+ * state, meant for benchmarks: with several threads sweeping at once the
+ * reported duration is that of whichever sweep recorded last. */
 void lsamd_set_timing(int enabled);
 double lsamd_last_sweep_kernel_ms(void);
 
 /* Tile shape of the sweep kernel: threads per workgroup and frequencies per
- * thread, among the shapes compiled for the form and variant (128..512
+ * thread, among the shapes compiled for the form and variant (64..512
  * threads, 1..16 frequencies).  Zero restores the built-in choice.  Returns
- * LSAMD_ERR_INVALID for shapes not compiled. */
+ * LSAMD_ERR_INVALID for shapes not compiled.  Process-wide setting (as is
+ * the choice lsamd_autotune makes); set it before sweeping from several
+ * threads, not while they run. */
 int lsamd_set_tile(int form, int variant, int threads, int freqs_per_thread);
 /* Tile shape lsamd_sweep would use now (for reports). */
 void lsamd_get_tile(int form, int variant, int dtype, int *threads, int *freqs_per_thread);
