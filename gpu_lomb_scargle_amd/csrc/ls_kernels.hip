@@ -86,6 +86,49 @@ __device__ __forceinline__ void peaks_insert(Peaks &L, double p, uint32_t i) {
     }
 }
 
+// Adds bin i to the list kept for top_k = 1 (the best bin only) or 3;
+// nothing for top_k = 0.  Each thread adds its bins in ascending order.
+__device__ __forceinline__ void peaks_add(Peaks &L, int top_k, double p, uint32_t i) {
+    if (top_k == 1) {
+        if (better(p, i, L.p[0], L.i[0])) {
+            L.p[0] = p;
+            L.i[0] = i;
+        }
+    } else if (top_k) {
+        peaks_insert(L, p, i);
+    }
+}
+
+// Best (power, bin) of a whole workgroup; the result is valid in thread 0.
+template <int T>
+__device__ __forceinline__ void block_best(double &p, uint32_t &i, double (*s_p)[kPeaks],
+                                           uint32_t (*s_i)[kPeaks]) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double op = __shfl_xor(p, off, 64);
+        const uint32_t oi = __shfl_xor(i, off, 64);
+        if (better(op, oi, p, i)) {
+            p = op;
+            i = oi;
+        }
+    }
+    if constexpr (T > 64) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) {
+            s_p[wave][0] = p;
+            s_i[wave][0] = i;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < T / 64; ++w)
+                if (better(s_p[w][0], s_i[w][0], p, i)) {
+                    p = s_p[w][0];
+                    i = s_i[w][0];
+                }
+        }
+    }
+}
+
 // Merge the lists of a whole workgroup; the result is valid in thread 0.
 template <int T>
 __device__ __forceinline__ void block_peaks(Peaks &L, double (*s_p)[kPeaks],
@@ -188,7 +231,7 @@ struct SweepArgs {
     void *pgram;
     double *part_pow;
     uint32_t *part_idx;
-    int want_peaks;
+    int top_k;                 // peaks kept per chunk: 0, 1 or 3
 };
 
 struct ReduceArgs {
@@ -354,9 +397,17 @@ template <typename OUT, int T>
 __device__ __forceinline__ void write_partial(const SweepArgs &a, Peaks &L, uint32_t obj,
                                               uint32_t chunk, double (*s_p)[kPeaks],
                                               uint32_t (*s_i)[kPeaks]) {
+    const uint64_t o = ((uint64_t)obj * a.chunks + chunk) * kPeaks;
+    if (a.top_k == 1) {  // entries 1 and 2 of the chunk's slot stay unused
+        block_best<T>(L.p[0], L.i[0], s_p, s_i);
+        if (threadIdx.x == 0) {
+            a.part_pow[o] = L.p[0];
+            a.part_idx[o] = L.i[0];
+        }
+        return;
+    }
     block_peaks<T>(L, s_p, s_i);
     if (threadIdx.x == 0) {
-        const uint64_t o = ((uint64_t)obj * a.chunks + chunk) * kPeaks;
 #pragma unroll
         for (int r = 0; r < kPeaks; ++r) {
             a.part_pow[o + r] = L.p[r];
@@ -435,10 +486,10 @@ __global__ __launch_bounds__(T) void k_sweep_plain_std(SweepArgs a) {
             pw = (R)((double)pw * scale);  // main.cu:1335 / 1350
             if (kf < a.nf) {
                 if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-                peaks_insert(L, (double)pw, kf);
+                peaks_add(L, a.top_k, (double)pw, kf);
             }
         }
-        if (a.want_peaks) write_partial<R, T>(a, L, obj, chunk, s_pp, s_pi);
+        if (a.top_k) write_partial<R, T>(a, L, obj, chunk, s_pp, s_pi);
     }
 }
 
@@ -550,10 +601,10 @@ __global__ __launch_bounds__(T) void k_sweep_plain_gen(SweepArgs a) {
             const R pw = (yc * yc / ccv + ys * ys / ssv) / yy;
             if (kf < a.nf) {
                 if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-                peaks_insert(L, (double)pw, kf);
+                peaks_add(L, a.top_k, (double)pw, kf);
             }
         }
-        if (a.want_peaks) write_partial<R, T>(a, L, obj, chunk, s_pp, s_pi);
+        if (a.top_k) write_partial<R, T>(a, L, obj, chunk, s_pp, s_pi);
     }
 }
 
@@ -683,6 +734,7 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
             Yb = a.obj_gen[kGen * obj + 1];
             YY = a.obj_gen[kGen * obj + 2];
         }
+        const double iW = 1.0 / Wf;  // one division per work item, not seven per bin
         OUT *pg = static_cast<OUT *>(a.pgram);
         Peaks L;
         peaks_init(L);
@@ -691,10 +743,10 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
             const uint32_t kf = kbase + m * T + tid;
             double P;
             if constexpr (GEN) {
-                const double C = acc[0][m] / Wf, S = acc[1][m] / Wf;
-                const double YC = acc[2][m] / Wf - Yb * C, YS = acc[3][m] / Wf - Yb * S;
-                const double CC = acc[4][m] / Wf - C * C, SS = acc[5][m] / Wf - S * S;
-                const double CS = acc[6][m] / Wf - C * S;
+                const double C = acc[0][m] * iW, S = acc[1][m] * iW;
+                const double YC = acc[2][m] * iW - Yb * C, YS = acc[3][m] * iW - Yb * S;
+                const double CC = acc[4][m] * iW - C * C, SS = acc[5][m] * iW - S * S;
+                const double CS = acc[6][m] * iW - C * S;
                 const double D = CC * SS - CS * CS;
                 P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
             } else {
@@ -706,10 +758,10 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
             const OUT pw = (OUT)(P * scale);
             if (kf < a.nf) {
                 if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-                peaks_insert(L, (double)pw, kf);
+                peaks_add(L, a.top_k, (double)pw, kf);
             }
         }
-        if (a.want_peaks) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+        if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
     }
 }
 
@@ -828,6 +880,7 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
             Yb = a.obj_gen[kGen * obj + 1];
             YY = a.obj_gen[kGen * obj + 2];
         }
+        const double iW = 1.0 / Wf;  // one division per work item, not seven per bin
         OUT *pg = static_cast<OUT *>(a.pgram);
         Peaks L;
         peaks_init(L);
@@ -836,11 +889,11 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
             const uint32_t kf = kbase + m * T + tid;
             double P;
             if constexpr (GEN) {
-                const double C = acc[0][m] / Wf, S = acc[1][m] / Wf;
-                const double YC = acc[2][m] / Wf - Yb * C, YS = acc[3][m] / Wf - Yb * S;
-                const double CCw = acc[4][m] / Wf;
+                const double C = acc[0][m] * iW, S = acc[1][m] * iW;
+                const double YC = acc[2][m] * iW - Yb * C, YS = acc[3][m] * iW - Yb * S;
+                const double CCw = acc[4][m] * iW;
                 const double CC = CCw - C * C, SS = (1.0 - CCw) - S * S;
-                const double CS = acc[5][m] / Wf - C * S;
+                const double CS = acc[5][m] * iW - C * S;
                 const double D = CC * SS - CS * CS;
                 P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
             } else {
@@ -852,10 +905,10 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
             const OUT pw = (OUT)(P * scale);
             if (kf < a.nf) {
                 if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-                peaks_insert(L, (double)pw, kf);
+                peaks_add(L, a.top_k, (double)pw, kf);
             }
         }
-        if (a.want_peaks) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+        if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
     }
 }
 
@@ -872,9 +925,9 @@ __global__ __launch_bounds__(64) void k_reduce_peaks(ReduceArgs a) {
     Peaks L;
     peaks_init(L);
     const uint64_t base = (uint64_t)k * a.chunks * kPeaks;
+    const int nr = a.top_k == 1 ? 1 : kPeaks;  // top-1 sweeps write entry 0 of each slot
     for (uint32_t c = threadIdx.x; c < a.chunks; c += 64)
-#pragma unroll
-        for (int r = 0; r < kPeaks; ++r)
+        for (int r = 0; r < nr; ++r)
             peaks_insert(L, a.part_pow[base + (uint64_t)c * kPeaks + r],
                          a.part_idx[base + (uint64_t)c * kPeaks + r]);
     block_peaks<64>(L, nullptr, nullptr);
@@ -1185,7 +1238,7 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     sw.pgram = a->pgram;
     sw.part_pow = reinterpret_cast<double *>(ws + w.part_pow);
     sw.part_idx = reinterpret_cast<uint32_t *>(ws + w.part_idx);
-    sw.want_peaks = a->top_k > 0 ? 1 : 0;
+    sw.top_k = (int)a->top_k;
 
     const uint64_t items = (uint64_t)K * w.chunks;
     sw.total_items = items;
