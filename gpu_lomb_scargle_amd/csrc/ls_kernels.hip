@@ -42,6 +42,7 @@ namespace {
 // 1000-object batch against 64 (9.4e-6 -> 7.0e-6) for ~1% of the time; the
 // generalized variant keeps 64 (its error is set by the FP32 phasors).
 constexpr uint32_t kFlushStd = 32, kFlushGen = 64;
+constexpr uint32_t kGroup = 4;  // work items per workgroup fetch, split forms (a power of two)
 constexpr int kPeaks = 3;      // peaks tracked per object (top-1 and top-3)
 constexpr int kGen = 8;        // doubles per object of split-generalized constants
 constexpr double kInv2Pi = 0.159154943091895335768883763372514362;  // 1/(2 pi)
@@ -213,8 +214,9 @@ struct SweepArgs {
     const uint64_t *offsets;
     const uint32_t *order;
     uint32_t K, nf, chunks;
-    uint64_t item_base;        // first item of this launch (one workgroup per item)
+    uint64_t item_base;        // first item of this launch
     uint64_t total_items;      // K * chunks
+    uint32_t group;            // consecutive items per workgroup fetch (a power of two)
     unsigned long long *queue; // persistent schedule: next item counter, else null
     // plain
     const void *t, *y, *w;
@@ -374,20 +376,23 @@ __device__ __forceinline__ void item_coords(const SweepArgs &a, uint64_t item, u
     obj = a.order ? a.order[pos] : pos;
 }
 
-// Work items: either one per workgroup (item_base + blockIdx.x), or, with a
-// queue, persistent workgroups that take the next item from a counter until
-// the items run out.  Items are numbered in processing order (longest object
-// first), chunk-minor.
+// Work items are numbered in processing order (longest object first),
+// chunk-minor, and taken in aligned groups of a.group consecutive items (the
+// next chunks of the same object, mostly): either one group per workgroup
+// (item_base + blockIdx.x * group), or, with a queue, persistent workgroups
+// that take the next group from a counter until the items run out.
 __device__ __forceinline__ uint64_t fetch_item(const SweepArgs &a, uint64_t *s_item) {
     __syncthreads();  // every thread is done with the previous item and its s_item read
-    if (threadIdx.x == 0) *s_item = atomicAdd(a.queue, 1ull);
+    if (threadIdx.x == 0) *s_item = atomicAdd(a.queue, (unsigned long long)a.group);
     __syncthreads();
     return *s_item;
 }
 __device__ __forceinline__ uint64_t first_item(const SweepArgs &a, uint64_t *s_item) {
-    return a.queue ? fetch_item(a, s_item) : a.item_base + blockIdx.x;
+    return a.queue ? fetch_item(a, s_item) : a.item_base + (uint64_t)blockIdx.x * a.group;
 }
-__device__ __forceinline__ uint64_t next_item(const SweepArgs &a, uint64_t *s_item) {
+__device__ __forceinline__ uint64_t next_item(const SweepArgs &a, uint64_t *s_item, uint64_t item) {
+    const uint64_t n = item + 1;
+    if (((n - a.item_base) & (uint64_t)(a.group - 1)) != 0) return n;  // same group
     return a.queue ? fetch_item(a, s_item) : ~0ull;
 }
 
@@ -428,7 +433,7 @@ __global__ __launch_bounds__(T) void k_sweep_plain_std(SweepArgs a) {
     __shared__ uint32_t s_pi[T / 64][kPeaks];
     __shared__ uint64_t s_item;
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
-         item = next_item(a, &s_item)) {
+         item = next_item(a, &s_item, item)) {
         uint32_t obj, chunk;
         item_coords(a, item, obj, chunk);
         const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
@@ -505,7 +510,7 @@ __global__ __launch_bounds__(T) void k_sweep_plain_gen(SweepArgs a) {
     __shared__ uint32_t s_pi[T / 64][kPeaks];
     __shared__ uint64_t s_item;
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
-         item = next_item(a, &s_item)) {
+         item = next_item(a, &s_item, item)) {
         uint32_t obj, chunk;
         item_coords(a, item, obj, chunk);
         const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
@@ -631,8 +636,9 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
     __shared__ double s_pp[T / 64][kPeaks];
     __shared__ uint32_t s_pi[T / 64][kPeaks];
     __shared__ uint64_t s_item;
+    uint32_t staged = ~0u;  // object whose records the LDS holds whole (at most T of them)
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
-         item = next_item(a, &s_item)) {
+         item = next_item(a, &s_item, item)) {
         uint32_t obj, chunk;
         item_coords(a, item, obj, chunk);
         const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
@@ -646,15 +652,20 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
     #pragma unroll
             for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
 
+        // the next chunk of an object of at most T observations reuses them
+        const bool reuse = obj == staged && end - beg <= (uint64_t)T;
+        staged = end - beg <= (uint64_t)T ? obj : ~0u;
         for (uint64_t b0 = beg; b0 < end; b0 += T) {
             const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
-            __syncthreads();
-            if (tid < cnt) {
-                s_t[tid] = a.rec_t[b0 + tid];
-                s_a[tid] = a.rec_a[b0 + tid];
-                if constexpr (GEN) s_n[tid] = a.rec_n[b0 + tid];
+            __syncthreads();  // also orders the previous item's peak merge before this one's
+            if (!reuse) {
+                if (tid < cnt) {
+                    s_t[tid] = a.rec_t[b0 + tid];
+                    s_a[tid] = a.rec_a[b0 + tid];
+                    if constexpr (GEN) s_n[tid] = a.rec_n[b0 + tid];
+                }
+                __syncthreads();
             }
-            __syncthreads();
             for (uint32_t j0 = 0; j0 < cnt; j0 += kF) {
                 const uint32_t j1 = (j0 + kF < cnt) ? j0 + kF : cnt;
                 float p[NS][M];
@@ -786,8 +797,9 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
     __shared__ double s_pp[T / 64][kPeaks];
     __shared__ uint32_t s_pi[T / 64][kPeaks];
     __shared__ uint64_t s_item;
+    uint32_t staged = ~0u;  // object whose records the LDS holds whole (at most T of them)
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
-         item = next_item(a, &s_item)) {
+         item = next_item(a, &s_item, item)) {
         uint32_t obj, chunk;
         item_coords(a, item, obj, chunk);
         const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
@@ -801,16 +813,21 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
     #pragma unroll
             for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
 
+        // the next chunk of an object of at most T observations reuses them
+        const bool reuse = obj == staged && end - beg <= (uint64_t)T;
+        staged = end - beg <= (uint64_t)T ? obj : ~0u;
         for (uint64_t b0 = beg; b0 < end; b0 += T) {
             const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
-            __syncthreads();
-            if (tid < cnt) {
-                s_t[tid] = a.rec_t[b0 + tid];
-                s_r[tid] = a.rec_r[b0 + tid];
-                const float4 r4 = a.rec_a[b0 + tid];
-                s_a[tid] = make_float2(r4.x, r4.y);
+            __syncthreads();  // also orders the previous item's peak merge before this one's
+            if (!reuse) {
+                if (tid < cnt) {
+                    s_t[tid] = a.rec_t[b0 + tid];
+                    s_r[tid] = a.rec_r[b0 + tid];
+                    const float4 r4 = a.rec_a[b0 + tid];
+                    s_a[tid] = make_float2(r4.x, r4.y);
+                }
+                __syncthreads();
             }
-            __syncthreads();
             for (uint32_t j = 0; j < cnt; ++j) {
                 const double tj = s_t[j];
                 const double2 st = s_r[j];
@@ -1243,6 +1260,10 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     const uint64_t items = (uint64_t)K * w.chunks;
     sw.total_items = items;
     sw.queue = nullptr;
+    // items per workgroup fetch: the split forms reuse the staged records of
+    // a small object across its consecutive chunks
+    sw.group = split ? kGroup : 1u;
+    const uint64_t groups = (items + sw.group - 1) / sw.group;
     const bool persistent = a->schedule == LSAMD_SCHEDULE_PERSISTENT;
     uint32_t pgrid = 0;
     if (persistent) {
@@ -1253,7 +1274,7 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
         LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void *>(te->fn), T, 0));
         pgrid = (uint32_t)std::max(1, cus * std::max(1, per_cu));
-        if ((uint64_t)pgrid > items) pgrid = (uint32_t)items;
+        if ((uint64_t)pgrid > groups) pgrid = (uint32_t)groups;
         hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, sw.queue);
         LS_HIP(hipGetLastError());
     }
@@ -1273,10 +1294,11 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
         LS_HIP(hipLaunchKernel(reinterpret_cast<const void *>(te->fn), dim3(pgrid), dim3(T), kargs,
                                0, st));
     } else {
-        const uint64_t max_items = (0xFFFFFFFFull / (uint64_t)T) - 1;
-        for (uint64_t base = 0; base < items; base += max_items) {
-            const uint64_t cnt = std::min<uint64_t>(max_items, items - base);
-            sw.item_base = base;
+        // launches of whole groups, under 2^32 threads each
+        const uint64_t max_groups = (0xFFFFFFFFull / (uint64_t)T) - 1;
+        for (uint64_t g = 0; g < groups; g += max_groups) {
+            const uint64_t cnt = std::min<uint64_t>(max_groups, groups - g);
+            sw.item_base = g * sw.group;
             void *kargs[] = {&sw};
             LS_HIP(hipLaunchKernel(reinterpret_cast<const void *>(te->fn), dim3((uint32_t)cnt),
                                    dim3(T), kargs, 0, st));
