@@ -43,6 +43,7 @@ namespace {
 // generalized variant keeps 64 (its error is set by the FP32 phasors).
 constexpr uint32_t kFlushStd = 32, kFlushGen = 64;
 constexpr uint32_t kGroup = 4;  // work items per workgroup fetch, split forms (a power of two)
+constexpr uint64_t kGroupMinItems = 65536;  // fewer items: one per fetch (e.g. a single object)
 constexpr int kPeaks = 3;      // peaks tracked per object (top-1 and top-3)
 constexpr int kGen = 8;        // doubles per object of split-generalized constants
 constexpr double kInv2Pi = 0.159154943091895335768883763372514362;  // 1/(2 pi)
@@ -1261,8 +1262,9 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     sw.total_items = items;
     sw.queue = nullptr;
     // items per workgroup fetch: the split forms reuse the staged records of
-    // a small object across its consecutive chunks
-    sw.group = split ? kGroup : 1u;
+    // a small object across its consecutive chunks, when there are enough
+    // items to keep every CU busy with a quarter as many workgroups
+    sw.group = (split && items >= kGroupMinItems) ? kGroup : 1u;
     const uint64_t groups = (items + sw.group - 1) / sw.group;
     const bool persistent = a->schedule == LSAMD_SCHEDULE_PERSISTENT;
     uint32_t pgrid = 0;
