@@ -1,28 +1,31 @@
 // HIP kernels of the MI355X-native Lomb-Scargle sweep and the stream-ordered
 // device entry point lsamd_sweep().
 //
-// Work decomposition: one workgroup per (object, frequency chunk).  A chunk
-// is T*M consecutive frequency bins; thread tid owns bins
-// chunk*T*M + m*T + tid (m < M).  The object's observations are staged
-// through LDS T at a time and read back as wave-wide broadcasts; all sums
-// live in registers.  Per-object normalisation and the argmax/top-3 peak
-// search are fused: every workgroup writes its chunk's best three bins and a
-// small reduction kernel merges the chunks of each object.
+// Work decomposition: work items (object, frequency chunk), one workgroup
+// each, or taken in groups of consecutive chunks.  A chunk is T*M
+// consecutive frequency bins; thread tid owns bins chunk*T*M + m*T + tid
+// (m < M).  The object's observations are staged through LDS T at a time and
+// read back as wave-wide broadcasts; all sums live in registers.
+// Per-object normalisation and the argmax/top-3 peak search are fused: every
+// workgroup writes its chunk's best bin (or best three) and a small
+// reduction kernel merges the chunks of each object.
 //
-// Two arithmetic forms:
-//   plain  - the reference's arithmetic (source/main.cu:1566-1599 standard,
-//            main.cu:1701-1777 generalized): DTYPE phase omega*t, accurate
-//            sin/cos, sequential DTYPE sums with no FMA contraction, the
-//            tau form of the power.  Reproduces the reference CPU path.
-//   split  - FP64 phase reduction (revolutions, rounded to nearest), FP32
-//            hardware sin/cos of the reduced phase, an FP32 rotation
-//            recurrence across the thread's M bins (angle step T bins),
-//            FP32 partial sums over 32-64 observations folded into FP64
-//            accumulators, and the tau-free FP64 power
-//              standard:    P = (SS XC^2 - 2 CS XC XS + CC XS^2) / (2 (CC SS - CS^2))
-//              generalized: P = (SS YC^2 - 2 CS YC YS + CC YS^2) / ((CC SS - CS^2) YY)
-//            (centred weighted sums), equal to the reference's tau form
-//            because the 2x2 least-squares projection is rotation invariant.
+// Three arithmetic forms:
+//   plain   - the reference's arithmetic (source/main.cu:1566-1599 standard,
+//             main.cu:1701-1777 generalized): DTYPE phase omega*t, accurate
+//             sin/cos, sequential DTYPE sums with no FMA contraction, the
+//             tau form of the power.  Reproduces the reference CPU path.
+//   split   - FP64 phase reduction (revolutions, rounded to nearest), FP32
+//             hardware sin/cos seed corrected in FP64, then the thread's M
+//             bins by an FP64 rotation and three-term recurrence (step T
+//             bins) with FP64 sums.
+//   split32 - the same seed in FP32, an FP32 rotation across the bins, and
+//             FP32 partial sums over 32-64 observations folded into FP64.
+// Both split forms use the tau-free FP64 power
+//     standard:    P = (SS XC^2 - 2 CS XC XS + CC XS^2) / (2 (CC SS - CS^2))
+//     generalized: P = (SS YC^2 - 2 CS YC YS + CC YS^2) / ((CC SS - CS^2) YY)
+// (centred weighted sums), equal to the reference's tau form because the
+// 2x2 least-squares projection is rotation invariant.
 
 #include <hip/hip_runtime.h>
 
@@ -38,9 +41,9 @@ namespace lsamd {
 namespace {
 
 // Observations per FP32 partial sum of the split32 sweep before it is folded
-// into FP64: 32 (standard) halves the worst relative power error of the
-// 1000-object batch against 64 (9.4e-6 -> 7.0e-6) for ~1% of the time; the
-// generalized variant keeps 64 (its error is set by the FP32 phasors).
+// into FP64.  Standard 32: on the 1000-object batch the worst relative power
+// error drops from 9.4e-6 (64) to 7.0e-6 for ~1% of the time.  Generalized
+// 64: even 16 stays above 1e-5 there (1.2e-5), at ~7% of the time.
 constexpr uint32_t kFlushStd = 32, kFlushGen = 64;
 constexpr uint32_t kGroup = 4;  // work items per workgroup fetch, split forms (a power of two)
 constexpr uint64_t kGroupMinItems = 65536;  // fewer items: one per fetch (e.g. a single object)
