@@ -328,6 +328,43 @@ def test_many_tiny_objects():
     assert np.array_equal(r["index"][:, 0], ref[3])
 
 
+@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT32])
+def test_periodogram_beyond_2_32_elements(form):
+    """K * Nf > 2^32 periodogram elements (4400 objects x 1e6 bins, FP32,
+    17.6 GB on the device; the reference's 32-bit periodWriteOffset wraps
+    there, kernel.cu:386, 521).  Rows on both sides of element 2^32 match the
+    same objects swept alone, bit for bit, and every object's device peak is
+    the first maximum of its own row."""
+    torch = pytest.importorskip("torch")
+    from gpu_lomb_scargle_amd import device
+    K, nf, n = 4400, 1_000_000, 6
+    assert K * nf > 2**32
+    rng = np.random.default_rng(77)
+    t = np.sort(rng.uniform(0, 1000, (K, n)), axis=1).ravel()
+    om = np.repeat(rng.uniform(2.0, 140.0, K), n)   # each object's own signal frequency
+    y = np.sin(om * t) + 0.1 * rng.standard_normal(K * n)
+    ids = np.repeat(np.arange(K, dtype=np.uint32), n)
+    kw = dict(min_freq=1.0, max_freq=150.0, num_freqs=nf, variant=_native.STANDARD, form=form,
+              top_k=1)
+    tt, yy, _, off = device.numpy_to_device(ids, t, y, None, F32)
+    plan = device.SweepPlan(tt, yy, None, off, **kw)
+    plan.run()
+    torch.cuda.synchronize()
+    res = plan.results()
+    pg = res["pgram"]
+    assert torch.equal(res["index"][:, 0].long(), torch.argmax(pg, dim=1))
+    edge = 2**32 // nf   # the row holding element 2^32
+    for k in (0, edge - 1, edge, edge + 1, K - 1):
+        a, b = k * n, (k + 1) * n
+        t1, y1, _, off1 = device.numpy_to_device(np.zeros(n, np.uint32), t[a:b], y[a:b], None, F32)
+        one = device.SweepPlan(t1, y1, None, off1, **kw)
+        one.run()
+        torch.cuda.synchronize()
+        assert torch.equal(one.results()["pgram"][0], pg[k]), k
+    del plan, res, pg
+    torch.cuda.empty_cache()
+
+
 def test_memory_budget_batching_is_exact():
     b = synth.lognormal_batch(50, 100, 0.5, 20, 400, 1800, True, 9)
     full = _native.batch_host(b["obj_id"], b["t"], b["y"], b["dy"], 1.0, 60.0, 3000,
