@@ -311,6 +311,68 @@ __global__ void k_obj_stats(StatsArgs a) {
     }
 }
 
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// The same statistics for the split forms, one wavefront per object: sums in
+// lane-strided order with a butterfly reduction (the split forms are not
+// bit-matched to the CPU path, and an object's sums no longer wait on one
+// thread's sequential loop over its observations).
+template <typename R>
+__global__ __launch_bounds__(64) void k_obj_stats_wave(StatsArgs a) {
+    const uint32_t k = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t b = a.offsets[k], e = a.offsets[k + 1];
+    const R *y = static_cast<const R *>(a.y);
+    const R *dy = static_cast<const R *>(a.dy);
+    const double nd = (double)(e - b);
+    double scale = 1.0;
+    if (a.normalize) {  // 2/(N sigma^2), sigma^2 the population variance
+        double s = 0.0;
+        for (uint64_t i = b + lane; i < e; i += 64) s += (double)y[i];
+        const double mean = wave_sum64(s) / nd;
+        double s2 = 0.0;
+        for (uint64_t i = b + lane; i < e; i += 64) s2 += ((double)y[i] - mean) * ((double)y[i] - mean);
+        const double sigma2 = wave_sum64(s2) / nd;
+        scale = 2.0 / (nd * sigma2);
+    }
+    if (lane == 0) a.obj_scale[k] = scale;
+    if (a.split_gen) {  // as k_obj_stats
+        double W = 0.0, WY = 0.0;
+        for (uint64_t i = b + lane; i < e; i += 64) {
+            const double wi = 1.0 / ((double)dy[i] * (double)dy[i]);
+            W += wi;
+            WY += wi * (double)y[i];
+        }
+        W = wave_sum64(W);
+        const double ybar = wave_sum64(WY) / W;
+        double Wf = 0.0, Ys = 0.0, YYs = 0.0;
+        for (uint64_t i = b + lane; i < e; i += 64) {
+            const double qi = sqrt(1.0 / ((double)dy[i] * (double)dy[i]) / W);
+            const double yc = (double)y[i] - ybar;
+            const double q = (double)(float)qi;
+            const float bq = (float)(qi * yc);
+            Wf += q * q;
+            Ys += (double)bq * q;
+            YYs += q * q * yc * yc;
+        }
+        Wf = wave_sum64(Wf);
+        Ys = wave_sum64(Ys);
+        YYs = wave_sum64(YYs);
+        if (lane == 0) {
+            const double Yb = Ys / Wf;
+            a.obj_gen[kGen * k + 0] = Wf;
+            a.obj_gen[kGen * k + 1] = Yb;
+            a.obj_gen[kGen * k + 2] = YYs / Wf - Yb * Yb;
+            a.obj_gen[kGen * k + 3] = ybar;
+            a.obj_gen[kGen * k + 4] = W;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Per-observation records.  One workgroup per object.
 // ---------------------------------------------------------------------------
@@ -1192,11 +1254,18 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     // 1. per-object statistics
     StatsArgs sa{a->offsets, a->y, a->dy, K, a->normalize && !gen, plain_pre ? 1 : 0,
                  (split && gen) ? 1 : 0, obj_shift, obj_scale, obj_gen};
-    const dim3 sb(128), sg((K + 127) / 128);
-    if (f64)
-        hipLaunchKernelGGL(k_obj_stats<double>, sg, sb, 0, st, sa);
-    else
-        hipLaunchKernelGGL(k_obj_stats<float>, sg, sb, 0, st, sa);
+    if (split) {
+        if (f64)
+            hipLaunchKernelGGL(k_obj_stats_wave<double>, dim3(K), dim3(64), 0, st, sa);
+        else
+            hipLaunchKernelGGL(k_obj_stats_wave<float>, dim3(K), dim3(64), 0, st, sa);
+    } else {
+        const dim3 sb(128), sg((K + 127) / 128);
+        if (f64)
+            hipLaunchKernelGGL(k_obj_stats<double>, sg, sb, 0, st, sa);
+        else
+            hipLaunchKernelGGL(k_obj_stats<float>, sg, sb, 0, st, sa);
+    }
     LS_HIP(hipGetLastError());
 
     // 2. per-observation records
