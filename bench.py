@@ -245,7 +245,9 @@ def uneven_batch(torch, L, dev, steps):
         a = _native.SweepArgs()
         ctypes_mod = __import__("ctypes")
         ctypes_mod.memmove(byref(a), byref(pers.args), ctypes_mod.sizeof(a))
-        a.num_objects, a.order, a.schedule = 1, None, _native.SCHEDULE_GRID
+        # the same kernels as the batch, one object per launch (no observation
+        # slices: a loop over objects is what the batch schedules replace)
+        a.num_objects, a.order, a.schedule, a.obs_slices = 1, None, _native.SCHEDULE_GRID, 1
         ws1 = torch.empty(int(L.lsamd_sweep_workspace_size(byref(a))), dtype=torch.uint8, device=dev)
         idx = torch.zeros(K * 3, dtype=torch.int32, device=dev)
         pw = torch.zeros(K * 3, dtype=torch.float64, device=dev)

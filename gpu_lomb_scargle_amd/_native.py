@@ -60,7 +60,7 @@ class SweepArgs(Structure):
         ("top_k", c_uint32), ("num_obs", c_uint64), ("min_freq", c_double), ("max_freq", c_double),
         ("offsets", c_void_p), ("order", c_void_p), ("t", c_void_p), ("y", c_void_p),
         ("dy", c_void_p), ("pgram", c_void_p), ("peak_index", c_void_p), ("peak_power", c_void_p),
-        ("peak_period", c_void_p), ("schedule", c_int32),
+        ("peak_period", c_void_p), ("schedule", c_int32), ("obs_slices", c_int32),
     ]
 
 

@@ -373,6 +373,9 @@ int lsamd_batch_host(int variant, int dtype, int form, int normalize, int precen
         a.y = c.y.p;
         a.dy = variant == LSAMD_GENERALIZED ? c.dy.p : nullptr;
         a.pgram = pgram ? static_cast<char *>(c.pgram.p) + batch_row(s0) * per_obj : nullptr;
+        // a call of one object may slice its observations; sub-sweeps of a
+        // larger call never do, so batching cannot change any result
+        a.obs_slices = K == 1 ? 0 : 1;
         if (tk) {
             a.peak_index = static_cast<uint32_t *>(c.pidx.p) + (size_t)s0 * tk;
             a.peak_power = static_cast<char *>(c.ppow.p) + (size_t)s0 * tk * rsz;

@@ -44,7 +44,9 @@ struct Workspace {
     size_t part_pow = 0;    // double[K*chunks*topk]
     size_t part_idx = 0;    // uint32[K*chunks*topk]
     size_t queue = 0;       // uint64   persistent-schedule item counter
+    size_t part_sums = 0;   // double[K*chunks*slices*NS*M*T] split forms with slices > 1
     uint32_t chunks = 0;
+    uint32_t slices = 1;    // observation slices per (object, chunk)
     Tile tile{};
 };
 int plan_workspace(const lsamd_sweep_args *a, Workspace *w);

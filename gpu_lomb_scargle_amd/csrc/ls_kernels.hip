@@ -47,6 +47,10 @@ namespace {
 constexpr uint32_t kFlushStd = 32, kFlushGen = 64;
 constexpr uint32_t kGroup = 4;  // work items per workgroup fetch, split forms (a power of two)
 constexpr uint64_t kGroupMinItems = 65536;  // fewer items: one per fetch (e.g. a single object)
+// Split forms: below this many (object, chunk) items, long objects are also
+// split by observations (at least T observations per slice), up to about
+// this many items.
+constexpr uint64_t kSliceTargetItems = 2048;
 constexpr int kPeaks = 3;      // peaks tracked per object (top-1 and top-3)
 constexpr int kGen = 8;        // doubles per object of split-generalized constants
 constexpr double kInv2Pi = 0.159154943091895335768883763372514362;  // 1/(2 pi)
@@ -221,6 +225,8 @@ struct SweepArgs {
     uint64_t item_base;        // first item of this launch
     uint64_t total_items;      // K * chunks
     uint32_t group;            // consecutive items per workgroup fetch (a power of two)
+    uint32_t slices;           // split forms: observation slices per (object, chunk), >= 1
+    double *part_sums;         // slices > 1: per-slice FP64 sums [K*chunks][slices][NS][M][T]
     unsigned long long *queue; // persistent schedule: next item counter, else null
     // plain
     const void *t, *y, *w;
@@ -440,6 +446,29 @@ __device__ __forceinline__ void item_coords(const SweepArgs &a, uint64_t item, u
     const uint32_t pos = (uint32_t)(item / a.chunks);
     chunk = (uint32_t)(item - (uint64_t)pos * a.chunks);
     obj = a.order ? a.order[pos] : pos;
+}
+
+// Split forms: item = (object position, chunk, observation slice), slice
+// minor.  With one slice an item is the whole object; with S slices it is
+// observations [beg + s L, beg + (s + 1) L) of the object, L = ceil(N / S).
+__device__ __forceinline__ void item_coords_sliced(const SweepArgs &a, uint64_t item, uint32_t &obj,
+                                                   uint32_t &chunk, uint32_t &slice, uint64_t &beg,
+                                                   uint64_t &end) {
+    uint64_t oc = item;
+    slice = 0;
+    if (a.slices > 1) {
+        oc = item / a.slices;
+        slice = (uint32_t)(item - oc * a.slices);
+    }
+    item_coords(a, oc, obj, chunk);
+    beg = a.offsets[obj];
+    end = a.offsets[obj + 1];
+    if (a.slices > 1) {
+        const uint64_t len = (end - beg + a.slices - 1) / a.slices;
+        const uint64_t b = beg + (uint64_t)slice * len;
+        beg = b < end ? b : end;
+        end = (beg + len) < end ? beg + len : end;
+    }
 }
 
 // Work items are numbered in processing order (longest object first),
@@ -680,6 +709,105 @@ __global__ __launch_bounds__(T) void k_sweep_plain_gen(SweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Split forms: the powers of a thread's M bins from the object's FP64 sums
+// (tau-free power, normalisation, periodogram store, the chunk's peaks).
+//   standard:    XC, XS, CC, CS; SS = N - CC
+//   generalized: C, S, YC, YS, CC, [SS,] CS over the normalised weights; SS
+//                summed (SS_SUM, split32) or 1 - CC (split, whose unit-modulus
+//                seeds make the weights sum to 1 to FP64 rounding)
+// ---------------------------------------------------------------------------
+template <bool SS_SUM, bool GEN, typename OUT, int T, int M, int NS>
+__device__ __forceinline__ void split_epilogue(const SweepArgs &a, const double (&acc)[NS][M],
+                                               uint32_t obj, uint32_t chunk,
+                                               double (*s_pp)[kPeaks], uint32_t (*s_pi)[kPeaks]) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t kbase = chunk * (uint32_t)(T * M);
+    const double scale = a.obj_scale[obj];
+    const double nobs = (double)(a.offsets[obj + 1] - a.offsets[obj]);
+    double Wf = 1.0, Yb = 0.0, YY = 1.0;
+    if constexpr (GEN) {
+        Wf = a.obj_gen[kGen * obj + 0];
+        Yb = a.obj_gen[kGen * obj + 1];
+        YY = a.obj_gen[kGen * obj + 2];
+    }
+    const double iW = 1.0 / Wf;  // one division per work item, not seven per bin
+    OUT *pg = static_cast<OUT *>(a.pgram);
+    Peaks L;
+    peaks_init(L);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const uint32_t kf = kbase + m * T + tid;
+        double P;
+        if constexpr (GEN) {
+            const double C = acc[0][m] * iW, S = acc[1][m] * iW;
+            const double YC = acc[2][m] * iW - Yb * C, YS = acc[3][m] * iW - Yb * S;
+            const double CCw = acc[4][m] * iW;
+            const double CC = CCw - C * C;
+            double SS, CS;
+            if constexpr (SS_SUM) {
+                SS = acc[5][m] * iW - S * S;
+                CS = acc[6][m] * iW - C * S;
+            } else {
+                SS = (1.0 - CCw) - S * S;
+                CS = acc[5][m] * iW - C * S;
+            }
+            const double D = CC * SS - CS * CS;
+            P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
+        } else {
+            const double XC = acc[0][m], XS = acc[1][m];
+            const double CC = acc[2][m], CS = acc[3][m], SS = nobs - CC;
+            const double D = CC * SS - CS * CS;
+            P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+        }
+        const OUT pw = (OUT)(P * scale);
+        if (kf < a.nf) {
+            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+            peaks_add(L, a.top_k, (double)pw, kf);
+        }
+    }
+    if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+}
+
+// One observation slice's sums of a thread's M bins, for k_combine_slices.
+template <int NS, int M, int T>
+__device__ __forceinline__ void store_slice(const SweepArgs &a, const double (&acc)[NS][M],
+                                            uint32_t obj, uint32_t chunk, uint32_t slice) {
+    double *dst = a.part_sums +
+                  (((uint64_t)obj * a.chunks + chunk) * a.slices + slice) * (uint64_t)(NS * M * T) +
+                  threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+        for (int m = 0; m < M; ++m) dst[(q * M + m) * T] = acc[q][m];
+}
+
+// Sums the observation slices of one (object, chunk) in slice order and
+// finishes it like the single-slice sweep.  One workgroup per (object, chunk).
+template <bool SS_SUM, bool GEN, typename OUT, int T, int M>
+__global__ __launch_bounds__(T) void k_combine_slices(SweepArgs a) {
+    constexpr int NS = GEN ? (SS_SUM ? 7 : 6) : 4;
+    __shared__ double s_pp[T / 64][kPeaks];
+    __shared__ uint32_t s_pi[T / 64][kPeaks];
+    const uint64_t oc = blockIdx.x;
+    const uint32_t obj = (uint32_t)(oc / a.chunks);
+    const uint32_t chunk = (uint32_t)(oc - (uint64_t)obj * a.chunks);
+    const double *src = a.part_sums + oc * a.slices * (uint64_t)(NS * M * T) + threadIdx.x;
+    double acc[NS][M];
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
+    for (uint32_t sl = 0; sl < a.slices; ++sl) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+#pragma unroll
+            for (int m = 0; m < M; ++m) acc[q][m] += src[(q * M + m) * T];
+        src += NS * M * T;
+    }
+    split_epilogue<SS_SUM, GEN, OUT, T, M, NS>(a, acc, obj, chunk, s_pp, s_pi);
+}
+
+// ---------------------------------------------------------------------------
 // Split form, FP32 sums ("split32").  Per observation and thread: one FP64
 // multiply and round for the seed phase (revolutions) of the thread's middle
 // bin, one hardware sin/cos pair, then the other M-1 bins by FP32 rotation
@@ -705,9 +833,9 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
     uint32_t staged = ~0u;  // object whose records the LDS holds whole (at most T of them)
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
          item = next_item(a, &s_item, item)) {
-        uint32_t obj, chunk;
-        item_coords(a, item, obj, chunk);
-        const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
+        uint32_t obj, chunk, slice;
+        uint64_t beg, end;
+        item_coords_sliced(a, item, obj, chunk, slice, beg, end);
         const uint32_t tid = threadIdx.x;
         const uint32_t kbase = chunk * (uint32_t)(T * M);
         const double fseed = a.fmin_rev + (double)(kbase + kSeed * T + tid) * a.df_rev;
@@ -719,8 +847,9 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
             for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
 
         // the next chunk of an object of at most T observations reuses them
-        const bool reuse = obj == staged && end - beg <= (uint64_t)T;
-        staged = end - beg <= (uint64_t)T ? obj : ~0u;
+        const bool whole = a.slices == 1 && end - beg <= (uint64_t)T;
+        const bool reuse = whole && obj == staged;
+        staged = whole ? obj : ~0u;
         for (uint64_t b0 = beg; b0 < end; b0 += T) {
             const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
             __syncthreads();  // also orders the previous item's peak merge before this one's
@@ -803,42 +932,11 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
             }
         }
 
-        const double scale = a.obj_scale[obj];
-        const double nobs = (double)(end - beg);
-        double Wf = 1.0, Yb = 0.0, YY = 1.0;
-        if constexpr (GEN) {
-            Wf = a.obj_gen[kGen * obj + 0];
-            Yb = a.obj_gen[kGen * obj + 1];
-            YY = a.obj_gen[kGen * obj + 2];
+        if (a.slices > 1) {
+            store_slice<NS, M, T>(a, acc, obj, chunk, slice);
+            continue;
         }
-        const double iW = 1.0 / Wf;  // one division per work item, not seven per bin
-        OUT *pg = static_cast<OUT *>(a.pgram);
-        Peaks L;
-        peaks_init(L);
-    #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const uint32_t kf = kbase + m * T + tid;
-            double P;
-            if constexpr (GEN) {
-                const double C = acc[0][m] * iW, S = acc[1][m] * iW;
-                const double YC = acc[2][m] * iW - Yb * C, YS = acc[3][m] * iW - Yb * S;
-                const double CC = acc[4][m] * iW - C * C, SS = acc[5][m] * iW - S * S;
-                const double CS = acc[6][m] * iW - C * S;
-                const double D = CC * SS - CS * CS;
-                P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
-            } else {
-                const double XC = acc[0][m], XS = acc[1][m];
-                const double CC = acc[2][m], CS = acc[3][m], SS = nobs - CC;
-                const double D = CC * SS - CS * CS;
-                P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
-            }
-            const OUT pw = (OUT)(P * scale);
-            if (kf < a.nf) {
-                if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-                peaks_add(L, a.top_k, (double)pw, kf);
-            }
-        }
-        if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+        split_epilogue<true, GEN, OUT, T, M, NS>(a, acc, obj, chunk, s_pp, s_pi);
     }
 }
 
@@ -866,9 +964,9 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
     uint32_t staged = ~0u;  // object whose records the LDS holds whole (at most T of them)
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
          item = next_item(a, &s_item, item)) {
-        uint32_t obj, chunk;
-        item_coords(a, item, obj, chunk);
-        const uint64_t beg = a.offsets[obj], end = a.offsets[obj + 1];
+        uint32_t obj, chunk, slice;
+        uint64_t beg, end;
+        item_coords_sliced(a, item, obj, chunk, slice, beg, end);
         const uint32_t tid = threadIdx.x;
         const uint32_t kbase = chunk * (uint32_t)(T * M);
         const double fseed = a.fmin_rev + (double)(kbase + tid) * a.df_rev;
@@ -880,8 +978,9 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
             for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
 
         // the next chunk of an object of at most T observations reuses them
-        const bool reuse = obj == staged && end - beg <= (uint64_t)T;
-        staged = end - beg <= (uint64_t)T ? obj : ~0u;
+        const bool whole = a.slices == 1 && end - beg <= (uint64_t)T;
+        const bool reuse = whole && obj == staged;
+        staged = whole ? obj : ~0u;
         for (uint64_t b0 = beg; b0 < end; b0 += T) {
             const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
             __syncthreads();  // also orders the previous item's peak merge before this one's
@@ -955,43 +1054,11 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
             }
         }
 
-        const double scale = a.obj_scale[obj];
-        const double nobs = (double)(end - beg);
-        double Wf = 1.0, Yb = 0.0, YY = 1.0;
-        if constexpr (GEN) {
-            Wf = a.obj_gen[kGen * obj + 0];
-            Yb = a.obj_gen[kGen * obj + 1];
-            YY = a.obj_gen[kGen * obj + 2];
+        if (a.slices > 1) {
+            store_slice<NS, M, T>(a, acc, obj, chunk, slice);
+            continue;
         }
-        const double iW = 1.0 / Wf;  // one division per work item, not seven per bin
-        OUT *pg = static_cast<OUT *>(a.pgram);
-        Peaks L;
-        peaks_init(L);
-    #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const uint32_t kf = kbase + m * T + tid;
-            double P;
-            if constexpr (GEN) {
-                const double C = acc[0][m] * iW, S = acc[1][m] * iW;
-                const double YC = acc[2][m] * iW - Yb * C, YS = acc[3][m] * iW - Yb * S;
-                const double CCw = acc[4][m] * iW;
-                const double CC = CCw - C * C, SS = (1.0 - CCw) - S * S;
-                const double CS = acc[5][m] * iW - C * S;
-                const double D = CC * SS - CS * CS;
-                P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
-            } else {
-                const double XC = acc[0][m], XS = acc[1][m];
-                const double CC = acc[2][m], CS = acc[3][m], SS = nobs - CC;
-                const double D = CC * SS - CS * CS;
-                P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
-            }
-            const OUT pw = (OUT)(P * scale);
-            if (kf < a.nf) {
-                if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-                peaks_add(L, a.top_k, (double)pw, kf);
-            }
-        }
-        if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+        split_epilogue<false, GEN, OUT, T, M, NS>(a, acc, obj, chunk, s_pp, s_pi);
     }
 }
 
@@ -1033,14 +1100,19 @@ typedef void (*SweepFn)(SweepArgs);
 struct TileEntry {
     int form, variant, dtype, T, M;
     SweepFn fn;
+    SweepFn combine;  // split forms: sums the observation slices (null: plain)
 };
 
-#define LS_SPLIT32(GEN, OUTT, DT, TT, MM) \
-    {LSAMD_FORM_SPLIT32, GEN, DT, TT, MM, k_sweep_split32<(GEN) != 0, OUTT, TT, MM>}
-#define LS_SPLIT(GEN, OUTT, DT, TT, MM) \
-    {LSAMD_FORM_SPLIT, GEN, DT, TT, MM, k_sweep_split64<(GEN) != 0, OUTT, TT, MM>}
-#define LS_PLAIN_STD(R, DT, TT, MM) {LSAMD_FORM_PLAIN, 0, DT, TT, MM, k_sweep_plain_std<R, TT, MM>}
-#define LS_PLAIN_GEN(R, DT, TT, MM) {LSAMD_FORM_PLAIN, 1, DT, TT, MM, k_sweep_plain_gen<R, TT, MM>}
+#define LS_SPLIT32(GEN, OUTT, DT, TT, MM)                                      \
+    {LSAMD_FORM_SPLIT32, GEN, DT, TT, MM, k_sweep_split32<(GEN) != 0, OUTT, TT, MM>, \
+     k_combine_slices<true, (GEN) != 0, OUTT, TT, MM>}
+#define LS_SPLIT(GEN, OUTT, DT, TT, MM)                                      \
+    {LSAMD_FORM_SPLIT, GEN, DT, TT, MM, k_sweep_split64<(GEN) != 0, OUTT, TT, MM>, \
+     k_combine_slices<false, (GEN) != 0, OUTT, TT, MM>}
+#define LS_PLAIN_STD(R, DT, TT, MM) \
+    {LSAMD_FORM_PLAIN, 0, DT, TT, MM, k_sweep_plain_std<R, TT, MM>, nullptr}
+#define LS_PLAIN_GEN(R, DT, TT, MM) \
+    {LSAMD_FORM_PLAIN, 1, DT, TT, MM, k_sweep_plain_gen<R, TT, MM>, nullptr}
 
 const TileEntry kTiles[] = {
     // split form (FP64 rotation and sums), both output types
@@ -1150,6 +1222,24 @@ int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
     const size_t parts = (size_t)K * w->chunks * kPeaks;
     w->part_pow = take(8 * parts);
     w->part_idx = take(4 * parts);
+    // Observation slices (split forms): a single object whose (object, chunk)
+    // items are too few to fill the GPU has its observations split over
+    // workgroups whose FP64 sums k_combine_slices adds up.  Only single-object
+    // sweeps are sliced, so an object's result never depends on which other
+    // objects share its batch.
+    w->slices = 1;
+    if (a->form != LSAMD_FORM_PLAIN && a->obs_slices == 0 && K == 1) {
+        const uint64_t base = (uint64_t)K * w->chunks;
+        const uint64_t nbar = (n + K - 1) / K;
+        const uint64_t by_items = (kSliceTargetItems + base - 1) / base;
+        const uint64_t by_obs = (nbar + tile.threads - 1) / tile.threads;
+        const uint64_t sl = std::min(by_items, by_obs);
+        if (base < kSliceTargetItems && sl >= 2) {
+            w->slices = (uint32_t)sl;
+            const int ns = a->variant == LSAMD_GENERALIZED ? 7 : 4;
+            w->part_sums = take(8 * base * sl * (size_t)ns * tile.fpt * tile.threads);
+        }
+    }
     w->queue = take(8);
     w->bytes = off + 256;
     return LSAMD_OK;
@@ -1230,6 +1320,8 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: nothing to compute (no pgram, top_k=0)");
     if (a->schedule != LSAMD_SCHEDULE_GRID && a->schedule != LSAMD_SCHEDULE_PERSISTENT)
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: bad schedule");
+    if (a->obs_slices != 0 && a->obs_slices != 1)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: obs_slices must be 0 or 1");
     Workspace w;
     plan_workspace(a, &w);
     if (!workspace || workspace_bytes < w.bytes)
@@ -1330,9 +1422,11 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     sw.part_idx = reinterpret_cast<uint32_t *>(ws + w.part_idx);
     sw.top_k = (int)a->top_k;
 
-    const uint64_t items = (uint64_t)K * w.chunks;
+    const uint64_t items = (uint64_t)K * w.chunks * w.slices;
     sw.total_items = items;
     sw.queue = nullptr;
+    sw.slices = w.slices;
+    sw.part_sums = w.slices > 1 ? reinterpret_cast<double *>(ws + w.part_sums) : nullptr;
     // items per workgroup fetch: the split forms reuse the staged records of
     // a small object across its consecutive chunks, when there are enough
     // items to keep every CU busy with a quarter as many workgroups
@@ -1377,6 +1471,12 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
             LS_HIP(hipLaunchKernel(reinterpret_cast<const void *>(te->fn), dim3((uint32_t)cnt),
                                    dim3(T), kargs, 0, st));
         }
+    }
+    if (w.slices > 1) {
+        if (!te->combine) return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: no slice combiner");
+        void *kargs[] = {&sw};
+        LS_HIP(hipLaunchKernel(reinterpret_cast<const void *>(te->combine),
+                               dim3((uint32_t)((uint64_t)K * w.chunks)), dim3(T), kargs, 0, st));
     }
     if (timing) {
         LS_HIP(hipEventRecord(g_ev[1], st));

@@ -41,12 +41,14 @@ class SweepPlan:
     t, y, dy: 1-D float32/float64 CUDA tensors (dy only for the generalized
     variant); offsets: int64 CUDA tensor of K+1 row offsets.  order (optional
     int32/uint32 CUDA tensor of K object numbers) is the processing order;
-    by default objects are processed longest first.
+    by default objects are processed longest first.  obs_slices=1 keeps a
+    single-object split-form sweep from spreading its observations over
+    workgroups (0, the default, lets it).
     """
 
     def __init__(self, t, y, dy, offsets, *, min_freq, max_freq, num_freqs, variant,
                  form=_native.FORM_SPLIT, normalize=True, precenter=True, pgram=True, top_k=1,
-                 order="longest_first", schedule=_native.SCHEDULE_GRID, lib=None):
+                 order="longest_first", schedule=_native.SCHEDULE_GRID, obs_slices=0, lib=None):
         if not t.is_cuda:
             raise RuntimeError("SweepPlan needs CUDA (HIP) tensors")
         self.lib = lib or _native.core()
@@ -84,6 +86,7 @@ class SweepPlan:
             a.peak_power = self.peak_power.data_ptr()
             a.peak_period = self.peak_period.data_ptr()
         a.schedule = int(schedule)
+        a.obs_slices = int(obs_slices)
         self.args = a
         ws = int(self.lib.lsamd_sweep_workspace_size(byref(a)))
         self.workspace = torch.empty(ws, dtype=torch.uint8, device=dev)

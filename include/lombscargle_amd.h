@@ -127,6 +127,9 @@ typedef struct lsamd_sweep_args {
     void *peak_power;        /* device, K*top_k, or NULL */
     void *peak_period;       /* device, K*top_k (days), or NULL */
     int32_t schedule;        /* enum lsamd_schedule */
+    int32_t obs_slices;      /* split forms, 0: a sweep of one object with many observations
+                              * spreads them over workgroups (results then differ from the
+                              * same object swept in a batch by rounding only); 1: never */
 } lsamd_sweep_args;
 
 /* Bytes of device workspace lsamd_sweep needs for these arguments. */
@@ -135,7 +138,10 @@ size_t lsamd_sweep_workspace_size(const lsamd_sweep_args *args);
 /* Enqueue the sweep on `stream` (NULL = default stream).  The workspace
  * must be device memory of at least lsamd_sweep_workspace_size() bytes and
  * stay untouched until the sweep completes.  Does not synchronise and does
- * not allocate. */
+ * not allocate.  Split forms with few (object, frequency chunk) work items
+ * and long objects (e.g. one object) also split the observations over
+ * workgroups and add their FP64 sums in a second kernel; the workspace size
+ * then includes those per-slice sums. */
 int lsamd_sweep(const lsamd_sweep_args *args, void *workspace, size_t workspace_bytes,
                 void *stream);
 

@@ -86,6 +86,29 @@ def test_sweep_argument_checks():
     assert L.lsamd_sweep(byref(a), c_void_p(4096), need, None) == _native.ERR_INVALID
 
 
+def test_observation_slices_workspace():
+    """A split-form sweep of one long object reserves per-slice FP64 sums;
+    with obs_slices = 1, or for a batch, it does not."""
+    L = _native.core()
+    a = _native.SweepArgs()
+    a.variant, a.dtype, a.form = _native.STANDARD, _native.F64, _native.FORM_SPLIT32
+    a.num_objects, a.num_freqs, a.num_obs, a.top_k = 1, 100000, 3555, 1
+    a.min_freq, a.max_freq = 1.005, 150.796
+    sliced = L.lsamd_sweep_workspace_size(byref(a))
+    a.obs_slices = 1
+    whole = L.lsamd_sweep_workspace_size(byref(a))
+    # 49 chunks of 256 x 8 bins, 14 slices of <= 256 observations, 4 sums
+    assert sliced - whole >= 49 * 14 * 4 * 8 * 256 * 8
+    a.obs_slices, a.num_objects = 0, 2
+    assert L.lsamd_sweep_workspace_size(byref(a)) < sliced
+    a.form, a.num_objects = _native.FORM_PLAIN, 1
+    assert L.lsamd_sweep_workspace_size(byref(a)) < whole
+    a.obs_slices = 2
+    a.offsets = a.t = a.y = 1  # never dereferenced: rejected before any launch
+    assert L.lsamd_sweep(byref(a), c_void_p(4096), 1 << 30, None) == _native.ERR_INVALID
+    assert b"obs_slices" in L.lsamd_last_error()
+
+
 def test_point_argument_checks():
     """lsamd_power_grad / lsamd_refine_peaks validate before any launch."""
     L = _native.core()

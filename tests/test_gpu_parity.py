@@ -244,6 +244,99 @@ def test_split_form_jd_scale_times():
 
 
 # --------------------------------------------------------------------------
+# observation slices: a few long objects spread over workgroups by
+# observations, their FP64 sums added by a second kernel
+# --------------------------------------------------------------------------
+def _one_object(variant, n=3555, seed=1000):
+    rng = np.random.default_rng(seed)
+    t = np.sort(rng.integers(0, 1801, n) + rng.uniform(0, 0.3, n))
+    y = np.sin(TWO_PI * t / 0.37 + 1.0) + rng.normal(0, 0.3, n)
+    dy = None
+    if variant == _native.GENERALIZED:
+        dy = rng.uniform(0.01, 0.2, n)
+        y = 17.0 + 0.3 * np.sin(TWO_PI * t / 0.37 + 1.0) + rng.normal(0, 1.0, n) * dy
+    return t, y, dy
+
+
+@pytest.mark.parametrize("variant", [_native.STANDARD, _native.GENERALIZED])
+@pytest.mark.parametrize("form", [_native.FORM_SPLIT, _native.FORM_SPLIT32])
+def test_observation_slices_single_object(form, variant):
+    """One object of 3555 observations over 20011 bins is too few (object,
+    chunk) items to fill the GPU, so its observations are split into slices
+    of at most one LDS stage.  Checked against the FP64 oracle and against
+    the same object swept whole inside a 600-object batch (enough items that
+    nothing is sliced): the two differ only in how the sums are grouped."""
+    torch = pytest.importorskip("torch")
+    from gpu_lomb_scargle_amd import device
+    nf = 20011
+    t, y, dy = _one_object(variant)
+    gen = variant == _native.GENERALIZED
+    ref = oracle.single(t, y.copy(), dy, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, gen, F64, nthreads=8)
+    kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=nf, variant=variant,
+              form=form, top_k=3)
+    n = len(t)
+    t1, y1, dy1, off1 = device.numpy_to_device(np.zeros(n, np.uint32), t, y, dy, F64)
+    one = device.SweepPlan(t1, y1, dy1, off1, **kw)
+    one.run()
+    torch.cuda.synchronize()
+    r1 = one.results()
+    copies = 600
+    ids = np.repeat(np.arange(copies, dtype=np.uint32), n)
+    tb, yb, dyb, offb = device.numpy_to_device(ids, np.tile(t, copies), np.tile(y, copies),
+                                               None if dy is None else np.tile(dy, copies), F64)
+    batch = device.SweepPlan(tb, yb, dyb, offb, **kw)
+    batch.run()
+    torch.cuda.synchronize()
+    rb = batch.results()
+    p1 = r1["pgram"][0].cpu().numpy()
+    pb = rb["pgram"].cpu().numpy()
+    assert np.array_equal(pb[0], pb[-1])       # whole-object sweeps are per-object exact
+    # sliced vs whole: FP64 sums regrouped (split) or FP32 partials cut at other
+    # observations (split32)
+    tol = 1e-12 if form == _native.FORM_SPLIT else 2e-6
+    assert_close(p1, pb[0], tol, 1e-3, tol)
+    assert np.array_equal(r1["index"].cpu().numpy()[0], rb["index"].cpu().numpy()[0])
+    if form == _native.FORM_SPLIT or not gen:
+        assert_close(p1, ref[0], 1e-5, 1e-3, 2e-6)
+    else:
+        assert_close(p1, ref[0], 1e-4, 1e-3, 2e-5)   # generalized FP32 sums: aliases
+
+
+@pytest.mark.parametrize("form", [_native.FORM_SPLIT, _native.FORM_SPLIT32])
+def test_observation_slices_schedules_and_graph(form):
+    """Sliced sweeps: the persistent schedule and graph replays give the grid
+    schedule's results bit for bit."""
+    torch = pytest.importorskip("torch")
+    from gpu_lomb_scargle_amd import device
+    t, y, dy = _one_object(_native.GENERALIZED, n=2000, seed=3)
+    t1, y1, dy1, off1 = device.numpy_to_device(np.zeros(len(t), np.uint32), t, y, dy, F64)
+    kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=30001,
+              variant=_native.GENERALIZED, form=form, top_k=3)
+    g = device.SweepPlan(t1, y1, dy1, off1, schedule=_native.SCHEDULE_GRID, **kw)
+    p = device.SweepPlan(t1, y1, dy1, off1, schedule=_native.SCHEDULE_PERSISTENT, **kw)
+    g.run()
+    p.run()
+    torch.cuda.synchronize()
+    rg, rp = g.results(), p.results()
+    for k in ("pgram", "index", "power", "period"):
+        assert torch.equal(rg[k], rp[k]), k
+    ref = {k: v.clone() for k, v in rg.items()}
+    s = torch.cuda.Stream()
+    g.capture_graph(s)
+    for rep in range(3):
+        g.pgram.fill_(-1.0)
+        torch.cuda.synchronize()
+        g.run(s)
+        s.synchronize()
+        out = g.results()
+        for k in ("pgram", "index", "power", "period"):
+            assert torch.equal(out[k], ref[k]), (rep, k)
+        p.run()
+        torch.cuda.synchronize()
+    g.destroy_graph()
+
+
+# --------------------------------------------------------------------------
 # fused peaks (argmax / top-3) against the reference rule
 # --------------------------------------------------------------------------
 @pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT, _native.FORM_SPLIT32])
