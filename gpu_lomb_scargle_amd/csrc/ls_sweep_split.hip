@@ -1,0 +1,407 @@
+// Split-form sweep kernels (split: FP64 rotation and sums; split32: FP32
+// rotation and FP32 partial sums folded into FP64), the observation-slice
+// combiner, and their tile registry entries.
+
+#include "ls_device.h"
+
+namespace lsamd {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Split forms: the powers of a thread's M bins from the object's FP64 sums
+// (tau-free power, normalisation, periodogram store, the chunk's peaks).
+//   standard:    XC, XS, CC, CS; SS = N - CC
+//   generalized: C, S, YC, YS, CC, [SS,] CS over the normalised weights; SS
+//                summed (SS_SUM, split32) or 1 - CC (split, whose unit-modulus
+//                seeds make the weights sum to 1 to FP64 rounding)
+// ---------------------------------------------------------------------------
+template <bool SS_SUM, bool GEN, typename OUT, int T, int M, int NS>
+__device__ __forceinline__ void split_epilogue(const SweepArgs &a, const double (&acc)[NS][M],
+                                               uint32_t obj, uint32_t chunk,
+                                               double (*s_pp)[kPeaks], uint32_t (*s_pi)[kPeaks]) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t kbase = chunk * (uint32_t)(T * M);
+    const double scale = a.obj_scale[obj];
+    const double nobs = (double)(a.offsets[obj + 1] - a.offsets[obj]);
+    double Wf = 1.0, Yb = 0.0, YY = 1.0;
+    if constexpr (GEN) {
+        Wf = a.obj_gen[kGen * obj + 0];
+        Yb = a.obj_gen[kGen * obj + 1];
+        YY = a.obj_gen[kGen * obj + 2];
+    }
+    const double iW = 1.0 / Wf;  // one division per work item, not seven per bin
+    OUT *pg = static_cast<OUT *>(a.pgram);
+    Peaks L;
+    peaks_init(L);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const uint32_t kf = kbase + m * T + tid;
+        double P;
+        if constexpr (GEN) {
+            const double C = acc[0][m] * iW, S = acc[1][m] * iW;
+            const double YC = acc[2][m] * iW - Yb * C, YS = acc[3][m] * iW - Yb * S;
+            const double CCw = acc[4][m] * iW;
+            const double CC = CCw - C * C;
+            double SS, CS;
+            if constexpr (SS_SUM) {
+                SS = acc[5][m] * iW - S * S;
+                CS = acc[6][m] * iW - C * S;
+            } else {
+                SS = (1.0 - CCw) - S * S;
+                CS = acc[5][m] * iW - C * S;
+            }
+            const double D = CC * SS - CS * CS;
+            P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
+        } else {
+            const double XC = acc[0][m], XS = acc[1][m];
+            const double CC = acc[2][m], CS = acc[3][m], SS = nobs - CC;
+            const double D = CC * SS - CS * CS;
+            P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+        }
+        const OUT pw = (OUT)(P * scale);
+        if (kf < a.nf) {
+            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+            peaks_add(L, a.top_k, (double)pw, kf);
+        }
+    }
+    if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+}
+
+// One observation slice's sums of a thread's M bins, for k_combine_slices.
+template <int NS, int M, int T>
+__device__ __forceinline__ void store_slice(const SweepArgs &a, const double (&acc)[NS][M],
+                                            uint32_t obj, uint32_t chunk, uint32_t slice) {
+    double *dst = a.part_sums +
+                  (((uint64_t)obj * a.chunks + chunk) * a.slices + slice) * (uint64_t)(NS * M * T) +
+                  threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+        for (int m = 0; m < M; ++m) dst[(q * M + m) * T] = acc[q][m];
+}
+
+// Sums the observation slices of one (object, chunk) in slice order and
+// finishes it like the single-slice sweep.  One workgroup per (object, chunk).
+template <bool SS_SUM, bool GEN, typename OUT, int T, int M>
+__global__ __launch_bounds__(T) void k_combine_slices(SweepArgs a) {
+    constexpr int NS = GEN ? (SS_SUM ? 7 : 6) : 4;
+    __shared__ double s_pp[T / 64][kPeaks];
+    __shared__ uint32_t s_pi[T / 64][kPeaks];
+    const uint64_t oc = blockIdx.x;
+    const uint32_t obj = (uint32_t)(oc / a.chunks);
+    const uint32_t chunk = (uint32_t)(oc - (uint64_t)obj * a.chunks);
+    const double *src = a.part_sums + oc * a.slices * (uint64_t)(NS * M * T) + threadIdx.x;
+    double acc[NS][M];
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
+    for (uint32_t sl = 0; sl < a.slices; ++sl) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+#pragma unroll
+            for (int m = 0; m < M; ++m) acc[q][m] += src[(q * M + m) * T];
+        src += NS * M * T;
+    }
+    split_epilogue<SS_SUM, GEN, OUT, T, M, NS>(a, acc, obj, chunk, s_pp, s_pi);
+}
+
+// ---------------------------------------------------------------------------
+// Split form, FP32 sums ("split32").  Per observation and thread: one FP64
+// multiply and round for the seed phase (revolutions) of the thread's middle
+// bin, one hardware sin/cos pair, then the other M-1 bins by FP32 rotation
+// up and down with the observation's precomputed step (cos, sin of T bins);
+// FP32 partial sums over 32 (standard) or 64 (generalized) observations are
+// folded into FP64 accumulators.  Every FP32 instruction of the inner loop is an all-VGPR v_fmac_f32 /
+// v_mul_f32 (the records carry -sin D so that no negated operand forces the
+// slower VOP3 encoding).
+// ---------------------------------------------------------------------------
+template <bool GEN, typename OUT, int T, int M>
+__global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
+    // standard: XC, XS, CC, CS (SS = N - CC); generalized: all seven sums
+    // (its centred SS - S^2 cancels, so SS is accumulated, not derived)
+    constexpr int NS = GEN ? 7 : 4;
+    constexpr int kSeed = (M - 1) / 2;  // the bin whose phasor is evaluated directly
+    constexpr uint32_t kF = GEN ? kFlushGen : kFlushStd;
+    __shared__ double s_t[T];
+    __shared__ float4 s_a[T];
+    __shared__ float s_n[GEN ? T : 1];
+    __shared__ double s_pp[T / 64][kPeaks];
+    __shared__ uint32_t s_pi[T / 64][kPeaks];
+    __shared__ uint64_t s_item;
+    uint32_t staged = ~0u;  // object whose records the LDS holds whole (at most T of them)
+    for (uint64_t item = first_item(a, &s_item); item < a.total_items;
+         item = next_item(a, &s_item, item)) {
+        uint32_t obj, chunk, slice;
+        uint64_t beg, end;
+        item_coords_sliced(a, item, obj, chunk, slice, beg, end);
+        const uint32_t tid = threadIdx.x;
+        const uint32_t kbase = chunk * (uint32_t)(T * M);
+        const double fseed = a.fmin_rev + (double)(kbase + kSeed * T + tid) * a.df_rev;
+
+        double acc[NS][M];
+    #pragma unroll
+        for (int q = 0; q < NS; ++q)
+    #pragma unroll
+            for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
+
+        // the next chunk of an object of at most T observations reuses them
+        const bool whole = a.slices == 1 && end - beg <= (uint64_t)T;
+        const bool reuse = whole && obj == staged;
+        staged = whole ? obj : ~0u;
+        for (uint64_t b0 = beg; b0 < end; b0 += T) {
+            const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+            __syncthreads();  // also orders the previous item's peak merge before this one's
+            if (!reuse) {
+                if (tid < cnt) {
+                    s_t[tid] = a.rec_t[b0 + tid];
+                    s_a[tid] = a.rec_a[b0 + tid];
+                    if constexpr (GEN) s_n[tid] = a.rec_n[b0 + tid];
+                }
+                __syncthreads();
+            }
+            for (uint32_t j0 = 0; j0 < cnt; j0 += kF) {
+                const uint32_t j1 = (j0 + kF < cnt) ? j0 + kF : cnt;
+                float p[NS][M];
+    #pragma unroll
+                for (int q = 0; q < NS; ++q)
+    #pragma unroll
+                    for (int m = 0; m < M; ++m) p[q][m] = 0.0f;
+    #pragma unroll 2
+                for (uint32_t j = j0; j < j1; ++j) {
+                    const double tj = s_t[j];
+                    const float4 r = s_a[j];
+                    const float nsd = GEN ? s_n[j] : r.y;  // -sin D
+                    double ph = fseed * tj;
+                    ph -= __builtin_rint(ph);
+                    const float phf = (float)ph;
+                    float c = __builtin_amdgcn_cosf(phf);
+                    float s = __builtin_amdgcn_sinf(phf);
+                    if constexpr (GEN) {
+                        // u = sqrt(w) z: the rotation is linear, and every
+                        // weighted sum becomes one FMA (w c = sqrt(w) u_c,
+                        // w c^2 = u_c^2, w y c = (sqrt(w) y) u_c)
+                        c *= r.y;
+                        s *= r.y;
+                    }
+                    auto add = [&](int m, float cv, float sv) {
+                        if constexpr (GEN) {
+                            p[0][m] = fmaf(r.y, cv, p[0][m]);
+                            p[1][m] = fmaf(r.y, sv, p[1][m]);
+                            p[2][m] = fmaf(r.x, cv, p[2][m]);
+                            p[3][m] = fmaf(r.x, sv, p[3][m]);
+                            p[4][m] = fmaf(cv, cv, p[4][m]);
+                            p[5][m] = fmaf(sv, sv, p[5][m]);
+                            p[6][m] = fmaf(cv, sv, p[6][m]);
+                        } else {
+                            p[0][m] = fmaf(r.x, cv, p[0][m]);
+                            p[1][m] = fmaf(r.x, sv, p[1][m]);
+                            p[2][m] = fmaf(cv, cv, p[2][m]);
+                            p[3][m] = fmaf(cv, sv, p[3][m]);
+                        }
+                    };
+                    // The seed sits at bin kSeed; the rotation walks up to
+                    // bin M-1 and down to bin 0, so no bin is more than
+                    // M/2 rotations from its seed.
+                    add(kSeed, c, s);
+                    {
+                        float cu = c, su = s;
+    #pragma unroll
+                        for (int m = kSeed + 1; m < M; ++m) {
+                            const float cn = fmaf(cu, r.z, su * nsd);
+                            const float sn = fmaf(su, r.z, cu * r.w);
+                            cu = cn;
+                            su = sn;
+                            add(m, cu, su);
+                        }
+                    }
+    #pragma unroll
+                    for (int m = kSeed - 1; m >= 0; --m) {
+                        const float cn = fmaf(c, r.z, s * r.w);
+                        const float sn = fmaf(s, r.z, c * nsd);
+                        c = cn;
+                        s = sn;
+                        add(m, c, s);
+                    }
+                }
+    #pragma unroll
+                for (int q = 0; q < NS; ++q)
+    #pragma unroll
+                    for (int m = 0; m < M; ++m) acc[q][m] += (double)p[q][m];
+            }
+        }
+
+        if (a.slices > 1) {
+            store_slice<NS, M, T>(a, acc, obj, chunk, slice);
+            continue;
+        }
+        split_epilogue<true, GEN, OUT, T, M, NS>(a, acc, obj, chunk, s_pp, s_pi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Split form, FP64 recurrence and sums ("split", the accurate fast form).  The
+// seed of each (thread, observation) is the hardware FP32 sin/cos of the
+// FP64-reduced phase, corrected to first order by the FP64 residual of that
+// phase's rounding to FP32; the M bins then follow in FP64 from the
+// observation's FP64 step (one rotation, then the three-term recurrence) and
+// accumulate directly in FP64.  This removes the FP32 summation error that
+// dominates at ill-conditioned (aliasing) frequencies.  The seed is brought
+// to unit modulus by one Newton step, so sum c^2 + sum s^2 equals the sum of
+// the weights to FP64 rounding and SS is not accumulated: SS = N - CC
+// (standard) or W - CC (generalized, normalised weights).
+// ---------------------------------------------------------------------------
+template <bool GEN, typename OUT, int T, int M>
+__global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
+    constexpr int NS = GEN ? 6 : 4;
+    __shared__ double s_t[T];
+    __shared__ double2 s_r[T];
+    __shared__ float2 s_a[T];
+    __shared__ double s_pp[T / 64][kPeaks];
+    __shared__ uint32_t s_pi[T / 64][kPeaks];
+    __shared__ uint64_t s_item;
+    uint32_t staged = ~0u;  // object whose records the LDS holds whole (at most T of them)
+    for (uint64_t item = first_item(a, &s_item); item < a.total_items;
+         item = next_item(a, &s_item, item)) {
+        uint32_t obj, chunk, slice;
+        uint64_t beg, end;
+        item_coords_sliced(a, item, obj, chunk, slice, beg, end);
+        const uint32_t tid = threadIdx.x;
+        const uint32_t kbase = chunk * (uint32_t)(T * M);
+        const double fseed = a.fmin_rev + (double)(kbase + tid) * a.df_rev;
+
+        double acc[NS][M];
+    #pragma unroll
+        for (int q = 0; q < NS; ++q)
+    #pragma unroll
+            for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
+
+        // the next chunk of an object of at most T observations reuses them
+        const bool whole = a.slices == 1 && end - beg <= (uint64_t)T;
+        const bool reuse = whole && obj == staged;
+        staged = whole ? obj : ~0u;
+        for (uint64_t b0 = beg; b0 < end; b0 += T) {
+            const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
+            __syncthreads();  // also orders the previous item's peak merge before this one's
+            if (!reuse) {
+                if (tid < cnt) {
+                    s_t[tid] = a.rec_t[b0 + tid];
+                    s_r[tid] = a.rec_r[b0 + tid];
+                    const float4 r4 = a.rec_a[b0 + tid];
+                    s_a[tid] = make_float2(r4.x, r4.y);
+                }
+                __syncthreads();
+            }
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const double tj = s_t[j];
+                const double2 st = s_r[j];
+                const float2 yw = s_a[j];
+                double ph = fseed * tj;
+                ph -= __builtin_rint(ph);
+                const float phf = (float)ph;
+                const float cf = __builtin_amdgcn_cosf(phf);
+                const float sf = __builtin_amdgcn_sinf(phf);
+                const double res = (ph - (double)phf) * kTwoPi;  // exact residual, radians
+                double c = fma(-res, (double)sf, (double)cf);
+                double s = fma(res, (double)cf, (double)sf);
+                // one Newton step towards |z| = 1: the hardware sin/cos leave
+                // |z|^2 - 1 ~ 1e-7, which SS = W - CC would otherwise carry
+                double nf = fma(-0.5, fma(c, c, s * s), 1.5);
+                const double yv = (double)yw.x;
+                const double qv = (double)yw.y;  // generalized: sqrt(w)
+                if constexpr (GEN) nf *= qv;     // u = sqrt(w) z, |u|^2 = w
+                c *= nf;
+                s *= nf;
+                auto accumulate = [&](int m, double cv, double sv) {
+                    if constexpr (GEN) {
+                        acc[0][m] = fma(qv, cv, acc[0][m]);
+                        acc[1][m] = fma(qv, sv, acc[1][m]);
+                        acc[2][m] = fma(yv, cv, acc[2][m]);
+                        acc[3][m] = fma(yv, sv, acc[3][m]);
+                        acc[4][m] = fma(cv, cv, acc[4][m]);
+                        acc[5][m] = fma(cv, sv, acc[5][m]);
+                    } else {
+                        acc[0][m] = fma(yv, cv, acc[0][m]);
+                        acc[1][m] = fma(yv, sv, acc[1][m]);
+                        acc[2][m] = fma(cv, cv, acc[2][m]);
+                        acc[3][m] = fma(cv, sv, acc[3][m]);
+                    }
+                };
+                // Bin 1 is bin 0 rotated by the observation's step D; bins m >= 2
+                // follow from the three-term recurrence z[m+1] = 2 cos(D) z[m] -
+                // z[m-1] (2 FMA per bin instead of 4).  In FP64 its error stays
+                // below 2e-14 over 16 bins for every D, including D near 0 and pi
+                // (the rounding of cos D only perturbs the angle by O(D^2) there).
+                const double twocd = 2.0 * st.x;
+                double cp = c, sp = s;
+                accumulate(0, c, s);
+                if constexpr (M > 1) {
+                    c = fma(cp, st.x, -sp * st.y);
+                    s = fma(sp, st.x, cp * st.y);
+                    accumulate(1, c, s);
+                }
+    #pragma unroll
+                for (int m = 2; m < M; ++m) {
+                    const double cn = fma(twocd, c, -cp);
+                    const double sn = fma(twocd, s, -sp);
+                    cp = c;
+                    sp = s;
+                    c = cn;
+                    s = sn;
+                    accumulate(m, c, s);
+                }
+            }
+        }
+
+        if (a.slices > 1) {
+            store_slice<NS, M, T>(a, acc, obj, chunk, slice);
+            continue;
+        }
+        split_epilogue<false, GEN, OUT, T, M, NS>(a, acc, obj, chunk, s_pp, s_pi);
+    }
+}
+
+}  // namespace
+
+#define LS_SPLIT32(GEN, OUTT, DT, TT, MM)                                      \
+    {LSAMD_FORM_SPLIT32, GEN, DT, TT, MM, k_sweep_split32<(GEN) != 0, OUTT, TT, MM>, \
+     k_combine_slices<true, (GEN) != 0, OUTT, TT, MM>}
+#define LS_SPLIT(GEN, OUTT, DT, TT, MM)                                      \
+    {LSAMD_FORM_SPLIT, GEN, DT, TT, MM, k_sweep_split64<(GEN) != 0, OUTT, TT, MM>, \
+     k_combine_slices<false, (GEN) != 0, OUTT, TT, MM>}
+
+const TileEntry kSplitTiles[] = {
+    // split form (FP64 rotation and sums), both output types
+    LS_SPLIT(0, float, LSAMD_F32, 256, 8), LS_SPLIT(0, double, LSAMD_F64, 256, 8),
+    LS_SPLIT(0, float, LSAMD_F32, 256, 4), LS_SPLIT(0, double, LSAMD_F64, 256, 4),
+    LS_SPLIT(0, float, LSAMD_F32, 256, 16), LS_SPLIT(0, double, LSAMD_F64, 256, 16),
+    LS_SPLIT(0, float, LSAMD_F32, 128, 8), LS_SPLIT(0, double, LSAMD_F64, 128, 8),
+    LS_SPLIT(0, float, LSAMD_F32, 512, 8), LS_SPLIT(0, double, LSAMD_F64, 512, 8),
+    LS_SPLIT(1, float, LSAMD_F32, 256, 4), LS_SPLIT(1, double, LSAMD_F64, 256, 4),
+    LS_SPLIT(1, float, LSAMD_F32, 256, 8), LS_SPLIT(1, double, LSAMD_F64, 256, 8),
+    LS_SPLIT(1, float, LSAMD_F32, 128, 8), LS_SPLIT(1, double, LSAMD_F64, 128, 8),
+    LS_SPLIT(0, float, LSAMD_F32, 128, 16), LS_SPLIT(0, double, LSAMD_F64, 128, 16),
+    LS_SPLIT(1, float, LSAMD_F32, 256, 2), LS_SPLIT(1, double, LSAMD_F64, 256, 2),
+    // split32 form (FP32 rotation and chunk sums)
+    LS_SPLIT32(0, float, LSAMD_F32, 256, 8), LS_SPLIT32(0, double, LSAMD_F64, 256, 8),
+    LS_SPLIT32(0, float, LSAMD_F32, 256, 4), LS_SPLIT32(0, double, LSAMD_F64, 256, 4),
+    LS_SPLIT32(0, float, LSAMD_F32, 256, 2), LS_SPLIT32(0, double, LSAMD_F64, 256, 2),
+    LS_SPLIT32(0, float, LSAMD_F32, 128, 4), LS_SPLIT32(0, double, LSAMD_F64, 128, 4),
+    LS_SPLIT32(0, float, LSAMD_F32, 512, 4), LS_SPLIT32(0, double, LSAMD_F64, 512, 4),
+    LS_SPLIT32(1, float, LSAMD_F32, 256, 4), LS_SPLIT32(1, double, LSAMD_F64, 256, 4),
+    LS_SPLIT32(1, float, LSAMD_F32, 256, 2), LS_SPLIT32(1, double, LSAMD_F64, 256, 2),
+    LS_SPLIT32(1, float, LSAMD_F32, 128, 4), LS_SPLIT32(1, double, LSAMD_F64, 128, 4),
+    LS_SPLIT32(0, float, LSAMD_F32, 256, 6), LS_SPLIT32(0, double, LSAMD_F64, 256, 6),
+    LS_SPLIT32(1, float, LSAMD_F32, 256, 3), LS_SPLIT32(1, double, LSAMD_F64, 256, 3),
+    LS_SPLIT(0, float, LSAMD_F32, 256, 12), LS_SPLIT(0, double, LSAMD_F64, 256, 12),
+    LS_SPLIT(0, float, LSAMD_F32, 256, 14), LS_SPLIT(0, double, LSAMD_F64, 256, 14),
+    LS_SPLIT32(0, float, LSAMD_F32, 128, 6), LS_SPLIT32(0, double, LSAMD_F64, 128, 6),
+    LS_SPLIT32(0, float, LSAMD_F32, 128, 8), LS_SPLIT32(0, double, LSAMD_F64, 128, 8),
+    LS_SPLIT32(0, float, LSAMD_F32, 64, 6), LS_SPLIT32(0, double, LSAMD_F64, 64, 6),
+    LS_SPLIT32(1, float, LSAMD_F32, 256, 6), LS_SPLIT32(1, double, LSAMD_F64, 256, 6),
+    LS_SPLIT32(1, float, LSAMD_F32, 128, 3), LS_SPLIT32(1, double, LSAMD_F64, 128, 3),
+};
+const int kSplitTileCount = (int)(sizeof(kSplitTiles) / sizeof(kSplitTiles[0]));
+
+}  // namespace lsamd
