@@ -458,8 +458,8 @@ def main():
                          "HBM byte counters), then one sweep of each form; no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(objects shard across GPUs without any exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (a batch would shard "
+              "by objects, one slice per GPU, with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     import torch
