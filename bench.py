@@ -100,15 +100,26 @@ def cpu_baseline(data, nf, sample_objects, threads):
     kind = "reference" if oracle.ref_available(False, np.float64, True, threads) else "port"
     t0 = time.perf_counter()
     if kind == "reference":
-        oracle.ref_batch(o, t, y, None, 1.005, 150.796, nf, False, np.float64, threads=threads)
+        res = oracle.ref_batch(o, t, y, None, 1.005, 150.796, nf, False, np.float64, threads=threads)
     else:
-        oracle.batch(o, t, y, None, 1.005, 150.796, nf, False, np.float64, nthreads=threads)
+        res = oracle.batch(o, t, y, None, 1.005, 150.796, nf, False, np.float64, nthreads=threads)
     dt = time.perf_counter() - t0
     terms = float(rows) * nf
     return {"value": terms / dt, "unit": "terms/s", "cores": threads, "kind": kind,
             "sample": f"{sample_objects} of the 1000 objects ({rows} rows) x {nf} freqs, FP64, "
                       f"reference MODE 4 (OpenMP dynamic over objects), {dt:.1f} s",
-            "objects_per_s": sample_objects / dt}
+            "objects_per_s": sample_objects / dt}, np.asarray(res[1], np.float64)
+
+
+def periods_vs_cpu(gpu_periods, cpu_periods):
+    """Best periods of the sampled objects: each GPU form against the
+    reference CPU batch mode on the same objects."""
+    out = {}
+    for name, p in gpu_periods.items():
+        p = np.asarray(p, np.float64)[:cpu_periods.size]
+        out[name] = {"identical": f"{int((p == cpu_periods).sum())}/{cpu_periods.size}",
+                     "max_rel_diff": float(np.max(np.abs(p - cpu_periods) / cpu_periods))}
+    return out
 
 
 def full_accuracy(torch, res, ref, K):
@@ -531,6 +542,10 @@ def main():
     for name, r in runs.items():
         r["acc"] = full_accuracy(torch, r["plan"].results(), pp, K)
         log(f"accuracy {name}: {r['acc']}")
+    # best periods of the objects the CPU baseline also runs (MODE 4 check)
+    sample_periods = {"fp64_plain": pp["period"][:args.cpu_objects, 0].cpu().numpy()}
+    for name, r in runs.items():
+        sample_periods[name] = r["plan"].results()["period"][:args.cpu_objects, 0].cpu().numpy()
 
     def meets(acc):
         return acc["max_rel_err_masked"] <= MAX_REL_ERR and acc["argmax_identical"] == f"{K}/{K}"
@@ -604,7 +619,8 @@ def main():
         out["end_to_end_pcie"] = end_to_end(torch, nf)
         log(f"end to end: {out['end_to_end_pcie']}")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(data, nf, args.cpu_objects, args.cpu_threads)
+        out["cpu_baseline"], cpu_periods = cpu_baseline(data, nf, args.cpu_objects, args.cpu_threads)
+        out["cpu_baseline"]["best_periods_vs_gpu"] = periods_vs_cpu(sample_periods, cpu_periods)
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
