@@ -141,6 +141,22 @@ def test_dropin_gpu_single_mode2(dtype, error, fname):
                                                      TWO_PI * 10.0, 4000, error, npt, nthreads=8)[0])
 
 
+@pytest.mark.parametrize("error", [False, True])
+@pytest.mark.parametrize("form", ["split", "split32"])
+def test_dropin_single_object_fast_forms(form, error):
+    """MODE 2 (one object of 3555 observations) with a fast form: the sweep
+    spreads the observations over workgroups (observation slices).  Against
+    the FP64 oracle at the split forms' tolerances."""
+    o, t, y, dy = load_data("8205_normalized_with_error.txt")
+    ids, per, pg = ls.lombscargle(o, t, y.copy(), 0.1, 10.0, error, "GPU", dy if error else None,
+                                  20000, "double", False, form=form)
+    ref = oracle.single(t, y, dy if error else None, TWO_PI * 0.1, TWO_PI * 10.0, 20000, error, F64,
+                        nthreads=8)[0]
+    loose = form == "split32" and error   # generalized FP32 sums
+    assert_close(pg[0], ref, 1e-4 if loose else 1e-5, 1e-3, 2e-5 if loose else 2e-6)
+    assert argmax_agrees(pg, ref[None, :], 1e-4) == 0
+
+
 @pytest.mark.parametrize("name,dtype", [("double", F64), ("float", F32)])
 def test_golden_fixtures_on_gpu(golden, name, dtype):
     o, t, y, dy = load_data("8205_normalized_with_error.txt")
