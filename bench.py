@@ -163,7 +163,7 @@ def fp32_batch(torch, L, dev, data, nf, steps):
     from gpu_lomb_scargle_amd import _native, device, synth
     K = int(data["counts"].size)
     terms = float(data["counts"].sum()) * nf
-    out = {"workload": "config-1 batch with FP32 inputs and FP32 periodogram", "terms_per_step": terms}
+    out = {"workload": "the 1000-object batch with FP32 inputs and an FP32 periodogram", "terms_per_step": terms}
     t32, y32, _, off = device.numpy_to_device(data["obj_id"], data["t"], data["y"], None, np.float32, dev)
     mk = lambda t, y, form: device.SweepPlan(t, y, None, off, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
                                              num_freqs=nf, variant=_native.STANDARD, form=form, top_k=1)

@@ -5,8 +5,8 @@ reference's CPU path, ``ls_oracle.c``) and, when it has been built, over the
 reference's own CPU functions compiled into ``oracle/_ref/``.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
-``bench.py`` import this package, and only as the checker: the GPU library
-never routes through it.
+``bench.py`` import this package, and only to check GPU results against: the
+GPU library never routes through it.
 
 Reference citations (``source/main.cu`` of mgowanlock/gpu_lomb_scargle):
   * ``batch``   -> lombscargleCPUBatch[Error], main.cu:1401-1543 (MODE 4)

@@ -1,8 +1,8 @@
 """FP64 reference gradients of the Lomb-Scargle power (test infrastructure).
 
 Only the tests, ``__graft_entry__.smoke()`` and the benchmark's checks use
-this module: it is the checker for the GPU derivative path, never the path
-itself.
+this module: it is what the GPU derivative path is checked against, never the
+path itself.
 
 The power is written here in the reference's own tau form and differentiated
 by torch autograd in float64 on the CPU -- an algebraically different route

@@ -4,7 +4,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  The GPU library never links, loads or calls
  * this file; only tests/, the smoke check and the benchmark's CPU-baseline
- * leg use it, as the checker.
+ * leg use it, to check GPU results against.
  *
  * How it is pinned (each check lives in tests/):
  *   - bit-for-bit against the reference's own CPU functions compiled from

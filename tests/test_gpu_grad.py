@@ -1,7 +1,7 @@
 """GPU derivative path (lsamd_power_grad, lsamd_refine_peaks, the autograd
 Function) against the FP64 CPU references.
 
-Checkers: oracle/grad_ref.py (torch float64 autograd of the reference's tau
+Checked against: oracle/grad_ref.py (torch float64 autograd of the reference's tau
 form, pinned in test_grad_ref.py) and central finite differences of the C
 oracle's power.  Tolerances (stated per test):
   * power: relative 1e-10;

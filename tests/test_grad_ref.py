@@ -1,7 +1,7 @@
 """Pins the FP64 reference gradients (oracle/grad_ref.py) on the CPU.
 
-The reference has no derivatives, so the checker for the GPU derivative
-kernel is pinned here against what the mathematics fixes:
+The reference has no derivatives, so the gradients the GPU derivative kernel
+is checked against are pinned here by what the mathematics fixes:
   * the value equals the C oracle's power (bit-equal to the reference CPU
     build) at the same frequency,
   * d/domega, d/dy, d/d(dy) equal central finite differences of the C
