@@ -111,15 +111,27 @@ __device__ __forceinline__ void peaks_insert(Peaks &L, double p, uint32_t i) {
 }
 
 // Adds bin i to the list kept for top_k = 1 (the best bin only) or 3;
-// nothing for top_k = 0.  Each thread adds its bins in ascending order.
+// nothing for top_k = 0.  Each thread adds its bins in ascending order, so an
+// equal power never displaces an earlier bin and a strict comparison of the
+// powers decides.
 __device__ __forceinline__ void peaks_add(Peaks &L, int top_k, double p, uint32_t i) {
     if (top_k == 1) {
-        if (better(p, i, L.p[0], L.i[0])) {
+        if (p > L.p[0]) {
             L.p[0] = p;
             L.i[0] = i;
         }
     } else if (top_k) {
-        peaks_insert(L, p, i);
+        const bool c0 = p > L.p[0], c1 = p > L.p[1], c2 = p > L.p[2];
+        const double p2 = c1 ? L.p[1] : (c2 ? p : L.p[2]);
+        const uint32_t i2 = c1 ? L.i[1] : (c2 ? i : L.i[2]);
+        const double p1 = c0 ? L.p[0] : (c1 ? p : L.p[1]);
+        const uint32_t i1 = c0 ? L.i[0] : (c1 ? i : L.i[1]);
+        L.p[0] = c0 ? p : L.p[0];
+        L.i[0] = c0 ? i : L.i[0];
+        L.p[1] = p1;
+        L.i[1] = i1;
+        L.p[2] = p2;
+        L.i[2] = i2;
     }
 }
 
@@ -149,6 +161,64 @@ __device__ __forceinline__ void block_best(double &p, uint32_t &i, double (*s_p)
                     p = s_p[w][0];
                     i = s_i[w][0];
                 }
+        }
+    }
+}
+
+// Best (power, bin) of a whole workgroup, valid in every thread; `slot`
+// selects the LDS entries used, so that consecutive calls need no barrier
+// between them.
+template <int T>
+__device__ __forceinline__ void block_best_all(double &p, uint32_t &i, double (*s_p)[kPeaks],
+                                               uint32_t (*s_i)[kPeaks], int slot) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double op = __shfl_xor(p, off, 64);
+        const uint32_t oi = __shfl_xor(i, off, 64);
+        if (better(op, oi, p, i)) {
+            p = op;
+            i = oi;
+        }
+    }
+    if constexpr (T > 64) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) {
+            s_p[wave][slot] = p;
+            s_i[wave][slot] = i;
+        }
+        __syncthreads();
+        p = s_p[0][slot];
+        i = s_i[0][slot];
+#pragma unroll
+        for (int w = 1; w < T / 64; ++w)
+            if (better(s_p[w][slot], s_i[w][slot], p, i)) {
+                p = s_p[w][slot];
+                i = s_i[w][slot];
+            }
+    }
+}
+
+// Top three (power, bin) of a whole workgroup from every thread's own
+// sorted top three: three rounds of "best head, then its owner drops it",
+// i.e. three scans that exclude the bins already taken.  The result is
+// returned in every thread.
+template <int T>
+__device__ __forceinline__ void block_top3(Peaks &L, double (*s_p)[kPeaks],
+                                           uint32_t (*s_i)[kPeaks], Peaks &W) {
+#pragma unroll
+    for (int r = 0; r < kPeaks; ++r) {
+        double p = L.p[0];
+        uint32_t i = L.i[0];
+        block_best_all<T>(p, i, s_p, s_i, r);
+        W.p[r] = p;
+        W.i[r] = i;
+        if (L.p[0] == p && L.i[0] == i) {  // the owner (or an empty (0, 0) head)
+            L.p[0] = L.p[1];
+            L.i[0] = L.i[1];
+            L.p[1] = L.p[2];
+            L.i[1] = L.i[2];
+            L.p[2] = 0.0;
+            L.i[2] = 0u;
         }
     }
 }
@@ -270,12 +340,13 @@ __device__ __forceinline__ void write_partial(const SweepArgs &a, Peaks &L, uint
         }
         return;
     }
-    block_peaks<T>(L, s_p, s_i);
+    Peaks W;
+    block_top3<T>(L, s_p, s_i, W);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int r = 0; r < kPeaks; ++r) {
-            a.part_pow[o + r] = L.p[r];
-            a.part_idx[o + r] = L.i[r];
+            a.part_pow[o + r] = W.p[r];
+            a.part_idx[o + r] = W.i[r];
         }
     }
 }
