@@ -110,13 +110,34 @@ __global__ __launch_bounds__(T) void k_combine_slices(SweepArgs a) {
 // ---------------------------------------------------------------------------
 // Split form, FP32 sums ("split32").  Per observation and thread: one FP64
 // multiply and round for the seed phase (revolutions) of the thread's middle
-// bin, one hardware sin/cos pair, then the other M-1 bins by FP32 rotation
-// up and down with the observation's precomputed step (cos, sin of T bins);
-// FP32 partial sums over 32 (standard) or 64 (generalized) observations are
-// folded into FP64 accumulators.  Every FP32 instruction of the inner loop is an all-VGPR v_fmac_f32 /
-// v_mul_f32 (the records carry -sin D so that no negated operand forces the
-// slower VOP3 encoding).
+// bin, one hardware sin/cos pair, then the other M-1 bins in FP32 from the
+// observation's precomputed step D (cos, sin of T bins); FP32 partial sums
+// over 32 (standard) or 64 (generalized) observations are folded into FP64
+// accumulators.
+//
+// The standard sweep takes the other bins from the seed in one of two ways,
+// chosen per observation (so uniformly across the workgroup):
+//   * |sin D| >= kRecMinSin: one rotation to the seed's upper neighbour, then
+//     the three-term recurrence z[m+1] = 2 cos D z[m] - z[m-1] up and down,
+//     one FMA per component and bin.  A rounding error of 2 cos D moves the
+//     recurrence's angle by that error / (2 sin D), hence the bound on sin D
+//     (about 84% of observations qualify).
+//   * otherwise: the rotation up and down (two multiplies and two FMAs per
+//     bin), whose error does not depend on D.
+// Staging puts each block's recurrence observations first, so both kinds run
+// as branch-free loops.  Each recurrence step is one v_fmac_f32 that writes
+// over the older of its two inputs: new = X latest + older with
+// X = -2 cos D g_old g_latest, which yields -g_old z[new], so the register
+// holds the phasor up to a sign fixed at compile time.  The quadratic sums do
+// not see that sign; the linear ones multiply by the negated weight instead.
+// The generalized sweep (fewer bins per thread) always rotates: there the
+// recurrence saved less than the partitioned staging costs.
+// Every FP32 instruction of the inner loop is an all-VGPR v_fmac_f32 /
+// v_mul_f32 (the staged records carry -sin D, -2 cos D and the negated
+// weights, so that no negated operand forces the slower VOP3 encoding).
 // ---------------------------------------------------------------------------
+constexpr float kRecMinSin = 0.25f;
+
 template <bool GEN, typename OUT, int T, int M>
 __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
     // standard: XC, XS, CC, CS (SS = N - CC); generalized: all seven sums
@@ -124,13 +145,18 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
     constexpr int NS = GEN ? 7 : 4;
     constexpr int kSeed = (M - 1) / 2;  // the bin whose phasor is evaluated directly
     constexpr uint32_t kF = GEN ? kFlushGen : kFlushStd;
+    constexpr int kWaves = T / 64;
+    constexpr bool kRec = !GEN;  // recurrence class in use
     __shared__ double s_t[T];
     __shared__ float4 s_a[T];
+    __shared__ float4 s_k[kRec ? T : 1];  // {2 cos D, -2 cos D, -r.x, -r.y}
     __shared__ float s_n[GEN ? T : 1];
+    __shared__ uint32_t s_cnt[2][kRec ? kWaves : 1];
     __shared__ double s_pp[T / 64][kPeaks];
     __shared__ uint32_t s_pi[T / 64][kPeaks];
     __shared__ uint64_t s_item;
     uint32_t staged = ~0u;  // object whose records the LDS holds whole (at most T of them)
+    uint32_t na = 0;        // recurrence observations at the head of the staged block
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
          item = next_item(a, &s_item, item)) {
         uint32_t obj, chunk, slice;
@@ -153,7 +179,7 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
         for (uint64_t b0 = beg; b0 < end; b0 += T) {
             const uint32_t cnt = (uint32_t)((end - b0) < (uint64_t)T ? (end - b0) : (uint64_t)T);
             __syncthreads();  // also orders the previous item's peak merge before this one's
-            if (!reuse) {
+            if (!reuse && !kRec) {
                 if (tid < cnt) {
                     s_t[tid] = a.rec_t[b0 + tid];
                     s_a[tid] = a.rec_a[b0 + tid];
@@ -161,15 +187,60 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
                 }
                 __syncthreads();
             }
+            if constexpr (kRec) {
+                if (!reuse) {
+                    // stage the block with its recurrence observations first
+                    // (each kind in index order)
+                    const bool in = tid < cnt;
+                    double rt = 0.0;
+                    float4 ra = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (in) {
+                        rt = a.rec_t[b0 + tid];
+                        ra = a.rec_a[b0 + tid];
+                    }
+                    const bool rec = in && fabsf(ra.w) >= kRecMinSin;
+                    const uint64_t m_rec = __ballot(rec);
+                    const uint64_t m_rot = __ballot(in) & ~m_rec;
+                    const uint32_t lane = tid & 63u, wave = tid >> 6;
+                    const uint64_t below = (1ull << lane) - 1ull;
+                    if (lane == 0) {
+                        s_cnt[0][wave] = (uint32_t)__popcll(m_rec);
+                        s_cnt[1][wave] = (uint32_t)__popcll(m_rot);
+                    }
+                    __syncthreads();
+                    uint32_t pre_rec = 0, pre_rot = 0, n_rec = 0;
+    #pragma unroll
+                    for (int w = 0; w < kWaves; ++w) {
+                        const uint32_t cr = s_cnt[0][w], co = s_cnt[1][w];
+                        if ((uint32_t)w < wave) {
+                            pre_rec += cr;
+                            pre_rot += co;
+                        }
+                        n_rec += cr;
+                    }
+                    if (in) {
+                        const uint32_t dst =
+                            rec ? pre_rec + (uint32_t)__popcll(m_rec & below)
+                                : n_rec + pre_rot + (uint32_t)__popcll(m_rot & below);
+                        const float k2 = ra.z + ra.z;
+                        s_t[dst] = rt;
+                        s_a[dst] = ra;
+                        s_k[dst] = make_float4(k2, -k2, -ra.x, -ra.y);
+                    }
+                    na = n_rec;
+                    __syncthreads();
+                }
+            }
             for (uint32_t j0 = 0; j0 < cnt; j0 += kF) {
                 const uint32_t j1 = (j0 + kF < cnt) ? j0 + kF : cnt;
+                const uint32_t jm = na < j0 ? j0 : (na > j1 ? j1 : na);
                 float p[NS][M];
     #pragma unroll
                 for (int q = 0; q < NS; ++q)
     #pragma unroll
                     for (int m = 0; m < M; ++m) p[q][m] = 0.0f;
-    #pragma unroll 2
-                for (uint32_t j = j0; j < j1; ++j) {
+                // one observation: [j0, jm) by the recurrence, [jm, j1) by rotation
+                auto body = [&](uint32_t j, auto recur) {
                     const double tj = s_t[j];
                     const float4 r = s_a[j];
                     const float nsd = GEN ? s_n[j] : r.y;  // -sin D
@@ -179,52 +250,90 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
                     float c = __builtin_amdgcn_cosf(phf);
                     float s = __builtin_amdgcn_sinf(phf);
                     if constexpr (GEN) {
-                        // u = sqrt(w) z: the rotation is linear, and every
+                        // u = sqrt(w) z: the steps are linear, and every
                         // weighted sum becomes one FMA (w c = sqrt(w) u_c,
                         // w c^2 = u_c^2, w y c = (sqrt(w) y) u_c)
                         c *= r.y;
                         s *= r.y;
                     }
-                    auto add = [&](int m, float cv, float sv) {
+                    // bin m's sums from a register pair holding g z[m]
+                    auto add = [&](int m, float cv, float sv, float wy, float wx) {
                         if constexpr (GEN) {
-                            p[0][m] = fmaf(r.y, cv, p[0][m]);
-                            p[1][m] = fmaf(r.y, sv, p[1][m]);
-                            p[2][m] = fmaf(r.x, cv, p[2][m]);
-                            p[3][m] = fmaf(r.x, sv, p[3][m]);
+                            p[0][m] = fmaf(wy, cv, p[0][m]);
+                            p[1][m] = fmaf(wy, sv, p[1][m]);
+                            p[2][m] = fmaf(wx, cv, p[2][m]);
+                            p[3][m] = fmaf(wx, sv, p[3][m]);
                             p[4][m] = fmaf(cv, cv, p[4][m]);
                             p[5][m] = fmaf(sv, sv, p[5][m]);
                             p[6][m] = fmaf(cv, sv, p[6][m]);
                         } else {
-                            p[0][m] = fmaf(r.x, cv, p[0][m]);
-                            p[1][m] = fmaf(r.x, sv, p[1][m]);
+                            p[0][m] = fmaf(wx, cv, p[0][m]);
+                            p[1][m] = fmaf(wx, sv, p[1][m]);
                             p[2][m] = fmaf(cv, cv, p[2][m]);
                             p[3][m] = fmaf(cv, sv, p[3][m]);
                         }
                     };
-                    // The seed sits at bin kSeed; the rotation walks up to
-                    // bin M-1 and down to bin 0, so no bin is more than
-                    // M/2 rotations from its seed.
-                    add(kSeed, c, s);
-                    {
-                        float cu = c, su = s;
+                    // The seed sits at bin kSeed; the steps walk up to bin
+                    // M-1 and down to bin 0, so no bin is more than M/2
+                    // steps from its seed.
+                    add(kSeed, c, s, r.y, r.x);
+                    const float cu = fmaf(c, r.z, s * nsd);  // bin kSeed + 1
+                    const float su = fmaf(s, r.z, c * r.w);
+                    if constexpr (kSeed + 1 < M) add(kSeed + 1, cu, su, r.y, r.x);
+                    if constexpr (kRec && decltype(recur)::value) {
+                        const float4 k = s_k[j];
+                        // (older, latest) pairs and their signs; each step
+                        // overwrites the older with X latest + older
+                        auto walk = [&](float c0, float s0, float c1, float s1, int m0, int dm,
+                                        int nsteps) {
+                            int g0 = 1, g1 = 1;
     #pragma unroll
-                        for (int m = kSeed + 1; m < M; ++m) {
-                            const float cn = fmaf(cu, r.z, su * nsd);
-                            const float sn = fmaf(su, r.z, cu * r.w);
-                            cu = cn;
-                            su = sn;
-                            add(m, cu, su);
+                            for (int i = 0; i < nsteps; ++i) {
+                                const float X = (g0 * g1 > 0) ? k.y : k.x;
+                                c0 = fmaf(X, c1, c0);
+                                s0 = fmaf(X, s1, s0);
+                                const int gn = -g0;
+                                const float ct = c1, st = s1;
+                                c1 = c0;
+                                s1 = s0;
+                                c0 = ct;
+                                s0 = st;
+                                g0 = g1;
+                                g1 = gn;
+                                if (gn > 0)
+                                    add(m0 + i * dm, c1, s1, r.y, r.x);
+                                else
+                                    add(m0 + i * dm, c1, s1, k.w, k.z);
+                            }
+                        };
+                        walk(c, s, cu, su, kSeed + 2, 1, M - kSeed - 2);
+                        walk(cu, su, c, s, kSeed - 1, -1, kSeed);
+                    } else {
+                        float cv = cu, sv = su;
+    #pragma unroll
+                        for (int m = kSeed + 2; m < M; ++m) {
+                            const float cn = fmaf(cv, r.z, sv * nsd);
+                            const float sn = fmaf(sv, r.z, cv * r.w);
+                            cv = cn;
+                            sv = sn;
+                            add(m, cv, sv, r.y, r.x);
+                        }
+    #pragma unroll
+                        for (int m = kSeed - 1; m >= 0; --m) {
+                            const float cn = fmaf(c, r.z, s * r.w);
+                            const float sn = fmaf(s, r.z, c * nsd);
+                            c = cn;
+                            s = sn;
+                            add(m, c, s, r.y, r.x);
                         }
                     }
-    #pragma unroll
-                    for (int m = kSeed - 1; m >= 0; --m) {
-                        const float cn = fmaf(c, r.z, s * r.w);
-                        const float sn = fmaf(s, r.z, c * nsd);
-                        c = cn;
-                        s = sn;
-                        add(m, c, s);
-                    }
+                };
+                if constexpr (kRec) {
+    #pragma unroll 2
+                    for (uint32_t j = j0; j < jm; ++j) body(j, std::true_type{});
                 }
+    #pragma unroll 2
+                for (uint32_t j = jm; j < j1; ++j) body(j, std::false_type{});
     #pragma unroll
                 for (int q = 0; q < NS; ++q)
     #pragma unroll

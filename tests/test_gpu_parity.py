@@ -524,7 +524,10 @@ def test_identical_times_zero_sum_gives_zero_power(form):
     r = _native.batch_host(None, t, y, None, 1.0, 5.0, 50, _native.STANDARD, F64, form,
                            normalize=False)
     g = r["pgram"][0]
-    noise = 1e-6 if form == _native.FORM_SPLIT32 else 1e-9  # FP32 sums leave ~1e-8
+    # split32: XC ~ 1e-8 from the FP32 sums, and D = N^2 c^2 (1 - |z|^2) with
+    # FP32 phasors whose |z|^2 - 1 is rounding (~1e-7, sometimes far less), so
+    # single bins reach ~1e-6 of a possible ~1.8
+    noise = 1e-4 if form == _native.FORM_SPLIT32 else 1e-9
     assert np.all(~np.isfinite(g) | (np.abs(g) < noise))
 
 
