@@ -9,8 +9,9 @@ peak reduction).
 
 Reported (one JSON line on stdout):
   value           freq*obs terms/s of the FP32-split form: split32 (FP64 phase
-                  reduction, FP32 hardware sin/cos seeds, FP32 rotation and
-                  32-observation FP32 partial sums folded into FP64) when it
+                  reduction, FP32 hardware sin/cos seeds, FP32 rotation or
+                  three-term recurrence across bins, 32-observation FP32
+                  partial sums folded into FP64) when it
                   meets max relative power error <= 1e-5 against the FP64 form
                   with identical best periods, else the split form (FP64
                   rotation and sums); config.form names the one reported
@@ -579,8 +580,8 @@ def main():
                 "kernel_ms": round(q["kms"], 3), "tile": "%dx%d" % q["tile"], "accuracy": q["acc"],
                 "meets_1e-5_and_argmax": meets(q["acc"])}
 
-    forms = {"split32": "FP64 phase reduction, FP32 hardware sin/cos seeds, FP32 rotation and 32-observation "
-                        "FP32 partial sums folded into FP64, FP64 power",
+    forms = {"split32": "FP64 phase reduction, FP32 hardware sin/cos seeds, FP32 rotation or three-term recurrence "
+                        "across bins, 32-observation FP32 partial sums folded into FP64, FP64 power",
              "split": "FP64 phase reduction, FP32 hardware sin/cos seeds, FP64 rotation recurrence and sums, "
                       "FP64 power"}
     out = {

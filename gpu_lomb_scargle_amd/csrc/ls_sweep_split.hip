@@ -233,7 +233,10 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
             }
             for (uint32_t j0 = 0; j0 < cnt; j0 += kF) {
                 const uint32_t j1 = (j0 + kF < cnt) ? j0 + kF : cnt;
-                const uint32_t jm = na < j0 ? j0 : (na > j1 ? j1 : na);
+                // recurrence observations in pairs (a lone one takes the
+                // rotation loop, which handles any observation)
+                uint32_t jm = na < j0 ? j0 : (na > j1 ? j1 : na);
+                jm = j0 + ((jm - j0) & ~1u);
                 float p[NS][M];
     #pragma unroll
                 for (int q = 0; q < NS; ++q)
@@ -329,8 +332,10 @@ __global__ __launch_bounds__(T) void k_sweep_split32(SweepArgs a) {
                     }
                 };
                 if constexpr (kRec) {
-    #pragma unroll 2
-                    for (uint32_t j = j0; j < jm; ++j) body(j, std::true_type{});
+                    for (uint32_t j = j0; j < jm; j += 2) {
+                        body(j, std::true_type{});
+                        body(j + 1, std::true_type{});
+                    }
                 }
     #pragma unroll 2
                 for (uint32_t j = jm; j < j1; ++j) body(j, std::false_type{});

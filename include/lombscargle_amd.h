@@ -76,9 +76,11 @@ enum lsamd_form {
      * power: within ~2e-6 relative of FP64 on the bins that carry power
      * (P >= 1e-3 Pmax), including aliasing frequencies */
     LSAMD_FORM_SPLIT = 1,
-    /* as SPLIT but with an FP32 rotation and FP32 sums over 32 (standard) or
-     * 64 (generalized) observations folded into FP64: the fastest form;
-     * within 1e-5 of FP64 on well-conditioned data (7.0e-6 on a 1000-object
+    /* as SPLIT but with FP32 steps across frequencies (rotation, and for the
+     * standard variant the three-term recurrence where |sin D| >= 0.25) and
+     * FP32 sums over 32 (standard) or 64 (generalized) observations folded
+     * into FP64: the fastest form;
+     * within 1e-5 of FP64 on well-conditioned data (7.5e-6 on a 1000-object
      * standard batch), up to ~3e-5 on generalized data and ~1e-4 at strong
      * aliases */
     LSAMD_FORM_SPLIT32 = 2
