@@ -9,19 +9,23 @@ peak reduction).
 
 Reported (one JSON line on stdout):
   value           freq*obs terms/s of the FP32-split form: split32 (FP64 phase
-                  reduction, FP32 hardware sin/cos seeds, FP32 rotation or
-                  three-term recurrence across bins, 32-observation FP32
-                  partial sums folded into FP64) when it
+                  reduction, FP32 hardware sin/cos seeds; sums on the FP16
+                  matrix cores as row x column factor products with
+                  head/tail splitting and FP32 accumulation folded into
+                  FP64) when it
                   meets max relative power error <= 1e-5 against the FP64 form
                   with identical best periods, else the split form (FP64
                   rotation and sums); config.form names the one reported
   split, split32  both forms: time, kernel time, full-size accuracy
   fp64_plain      the same workload in the reference's FP64 arithmetic
   objects_per_s   objects per second of the reported form
-  roofline        the reported form's sweep kernel: canonical VALU work per
+  roofline        the reported form's sweep kernel: algorithmic work per
                   launch / its duration (HIP events on the launch stream)
-                  against MI355X's published FP32 vector peak (157.3 TFLOP/s);
-                  traffic = HBM bytes per launch
+                  against the peak of the units it runs on: for the
+                  matrix-core tile 16 flops per term against the dense FP16
+                  matrix peak (2.5 PFLOP/s); for a VALU tile the canonical
+                  20 flop-eq per term against the FP32 vector peak
+                  (157.3 TFLOP/s); traffic = HBM bytes per launch
                   from the committed rocprofv3 counter summary
   cpu_baseline    the reference's own CPU batch path (mode 4, OpenMP) timed on
                   the host on a bounded object sample of the same workload
@@ -68,6 +72,16 @@ CYCLES_PER_TERM_GEN = CYCLES_PER_TERM + 2.0               # 12 (generalized)
 PEAK_LANE_CYCLES = 256 * 4 * 16 * 2.4e9
 PEAK_TFLOPS_NONPACKED = 2.0 * PEAK_LANE_CYCLES / 1e12   # 78.6
 PEAK_TFLOPS = 157.3
+# The standard split32 sweep's default tile runs on the matrix cores: each
+# term is two complex multiply-accumulates (Z1 = sum y e^{i w t}, Z2 = sum
+# e^{2 i w t}) = 16 flops, executed as FP16 products with head/tail splitting
+# (3 FP16 products per real product, 48 flops per term).  MI355X's published
+# dense FP16 matrix peak is 2.5 PFLOP/s.
+MFMA_TILE = (64, 16)
+VALU_TILE = (256, 8)   # the fastest split32 tile on the vector ALUs
+MFMA_FLOPS_PER_TERM = 16.0
+MFMA_EXEC_FLOPS_PER_TERM = 48.0
+PEAK_TFLOPS_MFMA_F16 = 2500.0
 MAX_REL_ERR = 1e-5
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_hbm_traffic_1000x1e6.txt")
 
@@ -512,12 +526,18 @@ def main():
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)  # 1 GiB
         dst = src.clone()
         torch.cuda.synchronize()
-        for form in (_native.FORM_SPLIT32, _native.FORM_SPLIT, _native.FORM_PLAIN):
+        for form, tile in ((_native.FORM_SPLIT32, None), (_native.FORM_SPLIT32, VALU_TILE),
+                           (_native.FORM_SPLIT, None), (_native.FORM_PLAIN, None)):
+            if tile:
+                L.lsamd_set_tile(form, _native.STANDARD, *tile)
             plan = make(form)
             plan.run()
             torch.cuda.synchronize()
+            if tile:
+                L.lsamd_set_tile(form, _native.STANDARD, 0, 0)
             del plan
-        log("profile-once: 1 GiB copy, one split32, one split and one FP64 plain sweep")
+        log("profile-once: 1 GiB copy, one split32 sweep with each tile (matrix-core, VALU 256x8), "
+            "one split and one FP64 plain sweep")
         del src, dst
         return
 
@@ -527,10 +547,17 @@ def main():
         return T.value, M.value
 
     runs = {}
-    for name, form in (("split32", _native.FORM_SPLIT32), ("split", _native.FORM_SPLIT)):
+    # split32 with its default (matrix-core) tile and with the VALU tile
+    for name, form, tile in (("split32", _native.FORM_SPLIT32, None),
+                             ("split32_valu", _native.FORM_SPLIT32, VALU_TILE),
+                             ("split", _native.FORM_SPLIT, None)):
+        if tile:
+            L.lsamd_set_tile(form, _native.STANDARD, *tile)
         plan = make(form)
         ms, kms = timed(plan, args.steps, args.warmup)
         T, M = tile_of(form)
+        if tile:
+            L.lsamd_set_tile(form, _native.STANDARD, 0, 0)
         runs[name] = {"plan": plan, "ms": ms, "kms": kms, "tile": (T, M)}
         log(f"{name}: {ms:.3f} ms/step, sweep kernel {kms:.3f} ms, tile {T}x{M}")
 
@@ -556,23 +583,44 @@ def main():
     T, M = r["tile"]
     value = terms / (r["ms"] / 1e3)
     kernel_terms_s = terms / (r["kms"] / 1e3)
-    achieved = kernel_terms_s * 2.0 * CYCLES_PER_TERM / 1e12
-    kname = "k_sweep_%s<false, double, %d, %d>" % ("split32" if chosen == "split32" else "split64", T, M)
+    valu_achieved = kernel_terms_s * 2.0 * CYCLES_PER_TERM / 1e12
+    mfma = chosen == "split32" and (T, M) == MFMA_TILE
+    if mfma:
+        kname = "k_sweep_mfma<double>"
+        achieved = kernel_terms_s * MFMA_FLOPS_PER_TERM / 1e12
+        bound = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS_MFMA_F16,
+                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS_MFMA_F16, 4),
+                 "work_per_term": f"{MFMA_FLOPS_PER_TERM:g} flops (two complex multiply-accumulates) on "
+                                  f"FP16 matrix cores; executed {MFMA_EXEC_FLOPS_PER_TERM:g} (FP16 head/tail "
+                                  f"split, 3 products per real product)",
+                 "executed_tflops": round(kernel_terms_s * MFMA_EXEC_FLOPS_PER_TERM / 1e12, 3),
+                 "frac_executed": round(kernel_terms_s * MFMA_EXEC_FLOPS_PER_TERM / 1e12
+                                        / PEAK_TFLOPS_MFMA_F16, 4),
+                 "roofline_terms_per_s": PEAK_TFLOPS_MFMA_F16 * 1e12 / MFMA_EXEC_FLOPS_PER_TERM,
+                 "valu_canonical": {"achieved": round(valu_achieved, 3), "peak": round(PEAK_TFLOPS, 2),
+                                    "frac": round(valu_achieved / PEAK_TFLOPS, 4),
+                                    "note": "the VALU-form roofline (20 flop-eq per term against the "
+                                            "FP32 vector peak), which the matrix-core tile exceeds"}}
+    else:
+        kname = "k_sweep_%s<false, double, %d, %d>" % ("split32" if chosen == "split32" else "split64", T, M)
+        achieved = valu_achieved
+        bound = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_TFLOPS, 2),
+                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS, 4),
+                 "peak_nonpacked_issue_model": round(PEAK_TFLOPS_NONPACKED, 2),
+                 "frac_nonpacked_issue_model": round(achieved / PEAK_TFLOPS_NONPACKED, 4),
+                 "work_per_term": f"{CYCLES_PER_TERM:g} VALU lane-cycles = {2 * CYCLES_PER_TERM:g} flop-eq "
+                                  f"(mul + sin + cos at 1/{TRANS_CYCLES:g} rate + 5 FMA), canonical",
+                 "roofline_terms_per_s": PEAK_TFLOPS * 1e12 / (2.0 * CYCLES_PER_TERM)}
     tr = measured_traffic(kname)
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_TFLOPS, 2),
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS, 4),
-                "peak_nonpacked_issue_model": round(PEAK_TFLOPS_NONPACKED, 2),
-                "frac_nonpacked_issue_model": round(achieved / PEAK_TFLOPS_NONPACKED, 4),
+    roofline = dict(bound)
+    roofline.update({
                 "traffic": tr["bytes"] if tr else None,
                 "kernel": kname, "kernel_ms": round(r["kms"], 4),
-                "work_per_term": f"{CYCLES_PER_TERM:g} VALU lane-cycles = {2 * CYCLES_PER_TERM:g} flop-eq "
-                                 f"(mul + sin + cos at 1/{TRANS_CYCLES:g} rate + 5 FMA), canonical",
-                "roofline_terms_per_s": PEAK_TFLOPS * 1e12 / (2.0 * CYCLES_PER_TERM),
                 "algorithmic_hbm_bytes": K * nf * 8 + rows * 16,
                 "pgram_write_GBps": round(K * nf * 8 / (r["kms"] / 1e3) / 1e9, 1),
                 "traffic_source": ("rocprofv3 FETCH_SIZE x%.1f + WRITE_SIZE x%.1f (copy-calibrated), %s"
                                    % (tr["fetch_factor"], tr["write_factor"], os.path.relpath(TRAFFIC_FILE, ROOT))
-                                   if tr else None)}
+                                   if tr else None)})
 
     def form_report(name):
         q = runs[name]
@@ -580,8 +628,11 @@ def main():
                 "kernel_ms": round(q["kms"], 3), "tile": "%dx%d" % q["tile"], "accuracy": q["acc"],
                 "meets_1e-5_and_argmax": meets(q["acc"])}
 
-    forms = {"split32": "FP64 phase reduction, FP32 hardware sin/cos seeds, FP32 rotation or three-term recurrence "
-                        "across bins, 32-observation FP32 partial sums folded into FP64, FP64 power",
+    forms = {"split32": "FP64 phase reduction, FP32 hardware sin/cos seeds; matrix-core tile: per 32x32 bins the "
+                        "sums as complex products of a row and a column factor on FP16 matrix cores (head/tail "
+                        "split, FP32 accumulation folded into FP64 every 256 observations), FP64 power",
+             "split32_valu": "FP64 phase reduction, FP32 hardware sin/cos seeds, FP32 rotation or three-term "
+                             "recurrence across bins, 32-observation FP32 partial sums folded into FP64, FP64 power",
              "split": "FP64 phase reduction, FP32 hardware sin/cos seeds, FP64 rotation recurrence and sums, "
                       "FP64 power"}
     out = {
@@ -597,6 +648,7 @@ def main():
         "objects_per_s": K / (r["ms"] / 1e3),
         "accuracy": r["acc"],
         "split32": form_report("split32"),
+        "split32_valu_tile": dict(form_report("split32_valu"), form_detail=forms["split32_valu"]),
         "split": form_report("split"),
         "fp64_plain": {"value": terms / (ms_plain / 1e3), "ms_per_step": round(ms_plain, 3),
                        "kernel_ms": round(kms_plain, 3),

@@ -38,6 +38,9 @@ struct SweepArgs {
     double *part_pow;
     uint32_t *part_idx;
     int top_k;                 // peaks kept per chunk: 0, 1 or 3
+    // matrix-core tile: per-object column-factor fragments and y scale
+    uint4 *colfac;
+    float *obj_ys;
 };
 
 typedef void (*SweepFn)(SweepArgs);
@@ -47,6 +50,7 @@ struct TileEntry {
     int form, variant, dtype, T, M;
     SweepFn fn;
     SweepFn combine;  // split forms: sums the observation slices (null: plain)
+    SweepFn prep;     // per-object set-up launched before the sweep (K x G workgroups of 64), or null
 };
 extern const TileEntry kPlainTiles[];  // ls_sweep_plain.hip
 extern const int kPlainTileCount;
@@ -66,6 +70,7 @@ constexpr uint64_t kGroupMinItems = 65536;  // fewer items: one per fetch (e.g. 
 // split by observations (at least T observations per slice), up to about
 // this many items.
 constexpr uint64_t kSliceTargetItems = 2048;
+constexpr int kColfacFrags = 12;  // matrix-core tile: column-factor fragments per 16 observations
 constexpr int kPeaks = 3;      // peaks tracked per object (top-1 and top-3)
 constexpr int kGen = 8;        // doubles per object of split-generalized constants
 constexpr double kInv2Pi = 0.159154943091895335768883763372514362;  // 1/(2 pi)

@@ -322,11 +322,11 @@ int g_tile_M[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
 
 // Built-in shapes: the fastest of the tile sweep on the 1000-object x 1e6
 // batches (MI355X): split 256x16 (standard) / 256x8 (generalized); split32
-// 256x8 (standard) / 256x4 (generalized).  lsamd_autotune re-picks per
-// workload.
+// 64x16, the matrix-core tile (standard; 256x8 is the fastest VALU tile) /
+// 256x4 (generalized).  lsamd_autotune re-picks per workload.
 Tile default_tile(int form, int variant) {
     if (form == LSAMD_FORM_SPLIT) return variant ? Tile{256, 8} : Tile{256, 16};
-    if (form == LSAMD_FORM_SPLIT32) return variant ? Tile{256, 4} : Tile{256, 8};
+    if (form == LSAMD_FORM_SPLIT32) return variant ? Tile{256, 4} : Tile{64, 16};
     return Tile{256, 1};
 }
 
@@ -376,6 +376,11 @@ int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
         w->rec_a = take(16 * n);
         w->rec_n = take(4 * n);
         w->rec_r = take(16 * n);
+    }
+    const TileEntry *te = find_tile(a->form, a->variant, a->dtype, tile.threads, tile.fpt);
+    if (te && te->prep) {  // matrix-core tile: y scales and column-factor fragments
+        w->obj_ys = take(4 * K);
+        w->colfac = take((size_t)16 * 64 * kColfacFrags * (n / 16 + K + 1));
     }
     const size_t parts = (size_t)K * w->chunks * kPeaks;
     w->part_pow = take(8 * parts);
@@ -579,6 +584,8 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     sw.part_pow = reinterpret_cast<double *>(ws + w.part_pow);
     sw.part_idx = reinterpret_cast<uint32_t *>(ws + w.part_idx);
     sw.top_k = (int)a->top_k;
+    sw.colfac = w.colfac ? reinterpret_cast<uint4 *>(ws + w.colfac) : nullptr;
+    sw.obj_ys = w.obj_ys ? reinterpret_cast<float *>(ws + w.obj_ys) : nullptr;
 
     const uint64_t items = (uint64_t)K * w.chunks * w.slices;
     sw.total_items = items;
@@ -613,6 +620,15 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
             LS_HIP(hipEventCreate(&g_ev[1]));
         }
         LS_HIP(hipEventRecord(g_ev[0], st));
+    }
+    if (te->prep) {  // per-object set-up of the tile, inside the timed sweep
+        // wavefronts per object: enough for the mean object's 16-observation
+        // blocks, up to 256 (one object of many observations)
+        const uint64_t blocks = (a->num_obs / 16 + K - 1) / K + 1;
+        const uint32_t gy = (uint32_t)std::min<uint64_t>(256, blocks);
+        void *kargs[] = {&sw};
+        LS_HIP(hipLaunchKernel(reinterpret_cast<const void *>(te->prep), dim3(K, gy), dim3(64), kargs,
+                               0, st));
     }
     if (persistent) {
         sw.item_base = 0;

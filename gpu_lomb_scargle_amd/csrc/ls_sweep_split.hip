@@ -476,6 +476,307 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Matrix-core tile of split32, standard variant.  A work item is a block of
+// 32 x 32 bins, bin = chunk * 1024 + 32 a + b.  Its sums are complex products
+// of a row factor (the frequency f_a of column 0) and a column factor (b df):
+//   Z1[a][b] = sum_j (y_j e^{2 pi i f_a t_j}) e^{2 pi i b df t_j} = XC + i XS
+//   Z2[a][b] = sum_j e^{4 pi i f_a t_j} e^{4 pi i b df t_j}          = C2 + i S2
+// with CC = (N + C2) / 2, CS = S2 / 2, SS = N - CC.  Each complex sum is four
+// real products over blocks of 16 observations, formed by
+// v_mfma_f32_32x32x16_f16 with FP32 accumulation.  Every factor, scaled by
+// 2^10, is split into an FP16 head h and a tail l = fp16((x - h) 2^11); head x
+// head and the two cross products carry ~22 bits (the dropped tail x tail
+// term is 2^-22), in two accumulators combined as (hh + 2^-11 x) 2^-20.  The
+// 2^10 scale keeps FP16 subnormal heads (which the VALU conversions and the
+// matrix core do not treat alike) to |x| < 2^-24.  Phases are reduced in FP64
+// and evaluated by the hardware sin/cos, as in the VALU split32 sweep.  The
+// factors of Z2 are divided by their squared modulus (hardware sin/cos leave
+// |z|^2 - 1 ~ 1e-7, and CC = (N + C2) / 2 assumes unit modulus).
+// The column factor depends only on the object: k_colfac writes its operand
+// fragments once per 16 observations, and y is scaled per object by a power
+// of two to max |y| in [0.5, 1) (FP16 range; undone exactly in the power).
+// Operand layout of 32x32x16 (checked on the device): lane l holds row (A) or
+// column (B) l % 32 and observations 8 (l / 32) .. +7 of the block; the
+// result holds column l % 32, rows 8 (v / 4) + 4 (l / 32) + v % 4, v < 16.
+// ---------------------------------------------------------------------------
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr float kFacScale = 1024.0f;  // 2^10 on each factor
+
+__device__ __forceinline__ void split16(float x, _Float16 &h, _Float16 &l) {
+    h = (_Float16)(fabsf(x) < 0x1p-14f ? 0.0f : x);  // no FP16 subnormal heads
+    l = (_Float16)((x - (float)h) * 2048.0f);
+}
+
+// First column-factor slot of an object: the slots of object k are
+// [floor(beg/16) + k, floor(beg/16) + k + ceil(N/16)), which never reach the
+// next object's floor((beg + N)/16) + k + 1.
+__device__ __forceinline__ uint64_t colfac_slot(uint64_t beg, uint32_t obj) { return (beg >> 4) + obj; }
+
+// Wavefronts (object, y) with y < gridDim.y: the y scale (y = 0), then
+// kColfacFrags fragments per 16 observations for blocks y, y + gridDim.y, ...:
+// v_r, v_i, -v_i and w_r, w_i, -w_i (w = v^2), head and tail.  Observations
+// past the object's end have v = 0.
+__global__ __launch_bounds__(64) void k_colfac(SweepArgs a) {
+    const uint32_t k = blockIdx.x, l = threadIdx.x;
+    const uint64_t beg = a.offsets[k], end = a.offsets[k + 1];
+    const uint32_t n = (uint32_t)(end - beg);
+    if (blockIdx.y == 0) {
+        float m = 0.0f;
+        for (uint64_t i = beg + l; i < end; i += 64) m = fmaxf(m, fabsf(a.rec_a[i].x));
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if (l == 0) {
+            int e = 0;
+            (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+            a.obj_ys[k] = m > 0.0f ? ldexpf(1.0f, -e) : 1.0f;
+        }
+    }
+    const double fb = (double)(l & 31u) * a.df_rev;
+    uint4 *dst = a.colfac + colfac_slot(beg, k) * (uint64_t)(kColfacFrags * 64) + l;
+    for (uint32_t q = blockIdx.y; q * 16u < n; q += gridDim.y) {
+        half8_t f[kColfacFrags];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            const uint32_t j = q * 16u + 8u * (l >> 5) + (uint32_t)v;
+            float cb = 0.0f, sb = 0.0f;
+            if (j < n) {
+                double pb = fb * a.rec_t[beg + j];
+                pb -= __builtin_rint(pb);
+                cb = __builtin_amdgcn_cosf((float)pb);
+                sb = __builtin_amdgcn_sinf((float)pb);
+            }
+            _Float16 h, t;
+            split16(cb * kFacScale, h, t);
+            f[0][v] = h;
+            f[1][v] = t;
+            split16(sb * kFacScale, h, t);
+            f[2][v] = h;
+            f[3][v] = t;
+            f[4][v] = -h;
+            f[5][v] = -t;
+            // w = v^2 / |v|^2 (1/n ~ 2 - n for n ~ 1)
+            const float ccb = cb * cb, ssb = sb * sb, gb = kFacScale * (2.0f - (ccb + ssb));
+            split16((ccb - ssb) * gb, h, t);
+            f[6][v] = h;
+            f[7][v] = t;
+            split16(2.0f * cb * sb * gb, h, t);
+            f[8][v] = h;
+            f[9][v] = t;
+            f[10][v] = -h;
+            f[11][v] = -t;
+        }
+#pragma unroll
+        for (int i = 0; i < kColfacFrags; ++i) {
+            uint4 u;
+            __builtin_memcpy(&u, &f[i], sizeof(u));
+            dst[(uint64_t)(q * kColfacFrags + i) * 64] = u;
+        }
+    }
+}
+
+#define LS_MFMA(ACC, A, B) ACC = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B, ACC, 0, 0, 0)
+
+template <typename OUT>
+__global__ __launch_bounds__(64) void k_sweep_mfma(SweepArgs a) {
+    constexpr int T = 64, M = 16;  // 64 threads, 1024 bins per item
+    constexpr uint32_t kFoldObs = 256;  // FP32 accumulation length
+    __shared__ double s_t[T];
+    __shared__ float s_y[T];
+    __shared__ double s_acc[4][16][T];  // FP64 sums XC, XS, C2, S2 of a lane's 16 bins
+    __shared__ double s_pp[1][kPeaks];
+    __shared__ uint32_t s_pi[1][kPeaks];
+    __shared__ uint64_t s_item;
+    const uint32_t l = threadIdx.x, rc = l & 31u, kh = l >> 5;
+    constexpr double kTail = 1.0 / 2048.0;
+    constexpr double kUnscale = 1.0 / ((double)kFacScale * (double)kFacScale);
+    for (uint64_t item = first_item(a, &s_item); item < a.total_items;
+         item = next_item(a, &s_item, item)) {
+        // (object, chunk, slice); slices hold a multiple of 16 observations
+        uint64_t oc = item;
+        uint32_t slice = 0, obj, chunk;
+        if (a.slices > 1) {
+            oc = item / a.slices;
+            slice = (uint32_t)(item - oc * a.slices);
+        }
+        item_coords(a, oc, obj, chunk);
+        const uint64_t obeg = a.offsets[obj], oend = a.offsets[obj + 1];
+        uint64_t beg = obeg, end = oend;
+        if (a.slices > 1) {
+            const uint64_t len = (((oend - obeg) + a.slices - 1) / a.slices + 15u) & ~(uint64_t)15u;
+            const uint64_t b = obeg + (uint64_t)slice * len;
+            beg = b < oend ? b : oend;
+            end = (beg + len) < oend ? beg + len : oend;
+        }
+        const float ys = a.obj_ys[obj];
+        const uint32_t kbase = chunk * (uint32_t)(T * M);
+        const double fa = a.fmin_rev + (double)(kbase + 32u * rc) * a.df_rev;  // row rc, column 0
+        const uint4 *cf = a.colfac +
+                          (colfac_slot(obeg, obj) + ((beg - obeg) >> 4)) * (uint64_t)(kColfacFrags * 64) + l;
+        f32x16_t z1r_hh = {}, z1r_x = {}, z1i_hh = {}, z1i_x = {};
+        f32x16_t z2r_hh = {}, z2r_x = {}, z2i_hh = {}, z2i_x = {};
+        // after every kFoldObs observations the FP32 accumulators move into
+        // FP64 sums in LDS (each lane its own entries, so no barrier)
+        bool folded = false;
+        auto fold = [&]() {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const double p0 = folded ? s_acc[0][v][l] : 0.0, p1 = folded ? s_acc[1][v][l] : 0.0;
+                const double p2 = folded ? s_acc[2][v][l] : 0.0, p3 = folded ? s_acc[3][v][l] : 0.0;
+                s_acc[0][v][l] = p0 + (double)z1r_hh[v] + kTail * (double)z1r_x[v];
+                s_acc[1][v][l] = p1 + (double)z1i_hh[v] + kTail * (double)z1i_x[v];
+                s_acc[2][v][l] = p2 + (double)z2r_hh[v] + kTail * (double)z2r_x[v];
+                s_acc[3][v][l] = p3 + (double)z2i_hh[v] + kTail * (double)z2i_x[v];
+            }
+            z1r_hh = f32x16_t{};
+            z1r_x = f32x16_t{};
+            z1i_hh = f32x16_t{};
+            z1i_x = f32x16_t{};
+            z2r_hh = f32x16_t{};
+            z2r_x = f32x16_t{};
+            z2i_hh = f32x16_t{};
+            z2i_x = f32x16_t{};
+            folded = true;
+        };
+        for (uint64_t g0 = beg; g0 < end; g0 += kFoldObs) {
+        const uint64_t gend = (end - g0) < kFoldObs ? end : g0 + kFoldObs;
+        for (uint64_t b0 = g0; b0 < gend; b0 += T) {
+            const uint32_t cnt = (uint32_t)((gend - b0) < (uint64_t)T ? (gend - b0) : (uint64_t)T);
+            __syncthreads();  // also orders the previous item's peak merge before this one's
+            if (l < cnt) {
+                s_t[l] = a.rec_t[b0 + l];
+                s_y[l] = a.rec_a[b0 + l].x * (ys * kFacScale);
+            }
+            __syncthreads();
+            for (uint32_t j0 = 0; j0 < cnt; j0 += 16, cf += kColfacFrags * 64) {
+                half8_t col[kColfacFrags];
+#pragma unroll
+                for (int i = 0; i < kColfacFrags; ++i) {
+                    const uint4 u = cf[i * 64];
+                    __builtin_memcpy(&col[i], &u, sizeof(u));
+                }
+                half8_t ar_h, ar_l, ai_h, ai_l, qr_h, qr_l, qi_h, qi_l;  // y u and u^2
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const uint32_t j = j0 + 8u * kh + (uint32_t)v;  // < T: j0 <= T - 16
+                    const bool ok = j < cnt;
+                    const double tj = s_t[j];
+                    const float yj = ok ? s_y[j] : 0.0f;
+                    double ph = fa * tj;
+                    ph -= __builtin_rint(ph);
+                    float c = __builtin_amdgcn_cosf((float)ph);
+                    float s = __builtin_amdgcn_sinf((float)ph);
+                    c = ok ? c : 0.0f;
+                    s = ok ? s : 0.0f;
+                    _Float16 h, t;
+                    split16(yj * c, h, t);
+                    ar_h[v] = h;
+                    ar_l[v] = t;
+                    split16(yj * s, h, t);
+                    ai_h[v] = h;
+                    ai_l[v] = t;
+                    const float cc = c * c, ss = s * s, g = kFacScale * (2.0f - (cc + ss));  // u^2/|u|^2
+                    split16((cc - ss) * g, h, t);
+                    qr_h[v] = h;
+                    qr_l[v] = t;
+                    split16(2.0f * c * s * g, h, t);
+                    qi_h[v] = h;
+                    qi_l[v] = t;
+                }
+                // Re Z1 = (y u)_r v_r - (y u)_i v_i,  Im Z1 = (y u)_r v_i + (y u)_i v_r
+                LS_MFMA(z1r_hh, ar_h, col[0]);
+                LS_MFMA(z1r_x, ar_h, col[1]);
+                LS_MFMA(z1r_x, ar_l, col[0]);
+                LS_MFMA(z1r_hh, ai_h, col[4]);
+                LS_MFMA(z1r_x, ai_h, col[5]);
+                LS_MFMA(z1r_x, ai_l, col[4]);
+                LS_MFMA(z1i_hh, ar_h, col[2]);
+                LS_MFMA(z1i_x, ar_h, col[3]);
+                LS_MFMA(z1i_x, ar_l, col[2]);
+                LS_MFMA(z1i_hh, ai_h, col[0]);
+                LS_MFMA(z1i_x, ai_h, col[1]);
+                LS_MFMA(z1i_x, ai_l, col[0]);
+                // the same for Z2 with u^2 and w
+                LS_MFMA(z2r_hh, qr_h, col[6]);
+                LS_MFMA(z2r_x, qr_h, col[7]);
+                LS_MFMA(z2r_x, qr_l, col[6]);
+                LS_MFMA(z2r_hh, qi_h, col[10]);
+                LS_MFMA(z2r_x, qi_h, col[11]);
+                LS_MFMA(z2r_x, qi_l, col[10]);
+                LS_MFMA(z2i_hh, qr_h, col[8]);
+                LS_MFMA(z2i_x, qr_h, col[9]);
+                LS_MFMA(z2i_x, qr_l, col[8]);
+                LS_MFMA(z2i_hh, qi_h, col[6]);
+                LS_MFMA(z2i_x, qi_h, col[7]);
+                LS_MFMA(z2i_x, qi_l, col[6]);
+            }
+        }
+        if (gend < end) fold();
+        }
+        // the item's FP64 sums of result entry v
+        auto sums = [&](int v, double &XC, double &XS, double &C2, double &S2) {
+            XC = (double)z1r_hh[v] + kTail * (double)z1r_x[v];
+            XS = (double)z1i_hh[v] + kTail * (double)z1i_x[v];
+            C2 = (double)z2r_hh[v] + kTail * (double)z2r_x[v];
+            S2 = (double)z2i_hh[v] + kTail * (double)z2i_x[v];
+            if (folded) {
+                XC += s_acc[0][v][l];
+                XS += s_acc[1][v][l];
+                C2 += s_acc[2][v][l];
+                S2 += s_acc[3][v][l];
+            }
+            XC *= kUnscale;
+            XS *= kUnscale;
+            C2 *= kUnscale;
+            S2 *= kUnscale;
+        };
+        const double iys = 1.0 / (double)ys;  // a power of two
+        if (a.slices > 1) {
+            // the slice's FP64 sums in the layout k_combine_slices reads:
+            // XC, XS, CC, CS of bin m T + tid at [(q M + m) T + tid]
+            double *dst = a.part_sums + (((uint64_t)obj * a.chunks + chunk) * a.slices + slice) *
+                                            (uint64_t)(4 * M * T);
+            const double ns = (double)(end - beg);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t row = 8u * (uint32_t)(v / 4) + 4u * kh + (uint32_t)(v % 4);
+                const uint32_t bl = 32u * row + rc, m = bl / T, tt = bl % T;
+                double XC, XS, C2, S2;
+                sums(v, XC, XS, C2, S2);
+                dst[(0 * M + m) * T + tt] = XC * iys;
+                dst[(1 * M + m) * T + tt] = XS * iys;
+                dst[(2 * M + m) * T + tt] = 0.5 * (ns + C2);
+                dst[(3 * M + m) * T + tt] = 0.5 * S2;
+            }
+            continue;
+        }
+        const double scale = a.obj_scale[obj] * iys * iys;
+        const double nobs = (double)(oend - obeg);
+        OUT *pg = static_cast<OUT *>(a.pgram);
+        Peaks L;
+        peaks_init(L);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const uint32_t row = 8u * (uint32_t)(v / 4) + 4u * kh + (uint32_t)(v % 4);
+            const uint32_t kf = kbase + 32u * row + rc;  // ascending in v
+            double XC, XS, C2, S2;
+            sums(v, XC, XS, C2, S2);
+            const double CC = 0.5 * (nobs + C2), SS = nobs - CC, CS = 0.5 * S2;
+            const double D = CC * SS - CS * CS;
+            const double P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+            const OUT pw = (OUT)(P * scale);
+            if (kf < a.nf) {
+                if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+                peaks_add(L, a.top_k, (double)pw, kf);
+            }
+        }
+        if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+    }
+}
+
 }  // namespace
 
 #define LS_SPLIT32(GEN, OUTT, DT, TT, MM)                                      \
@@ -486,6 +787,11 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
      k_combine_slices<false, (GEN) != 0, OUTT, TT, MM>}
 
 const TileEntry kSplitTiles[] = {
+    // split32 on the matrix cores (standard variant): 64 threads, 32 x 32 bins
+    {LSAMD_FORM_SPLIT32, 0, LSAMD_F32, 64, 16, k_sweep_mfma<float>,
+     k_combine_slices<true, false, float, 64, 16>, k_colfac},
+    {LSAMD_FORM_SPLIT32, 0, LSAMD_F64, 64, 16, k_sweep_mfma<double>,
+     k_combine_slices<true, false, double, 64, 16>, k_colfac},
     // split form (FP64 rotation and sums), both output types
     LS_SPLIT(0, float, LSAMD_F32, 256, 8), LS_SPLIT(0, double, LSAMD_F64, 256, 8),
     LS_SPLIT(0, float, LSAMD_F32, 256, 4), LS_SPLIT(0, double, LSAMD_F64, 256, 4),

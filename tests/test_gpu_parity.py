@@ -419,16 +419,56 @@ def test_ragged_grids_and_staging_boundaries(form, nf):
         assert_close(pg[ok], ref[0][ok], 1e-5, 1e-3, 2e-6)
     else:
         # split32 derives SS = N - CC from FP32 phasors whose modulus drifts by
-        # ~1e-7 (hardware sin/cos, FP32 rotation).  On the 5-observation object
+        # ~1e-7 (hardware sin/cos; the VALU tile's FP32 rotation, the matrix-
+        # core tile's FP16 head/tail factors).  On the 5-observation object
         # the 2x2 system is nearly singular at some frequencies, and on the
-        # finest grid its error reaches ~1.2e-5 relative and ~8e-6 P_max (a CPU
-        # emulation with correctly rounded sin/cos puts that object at 6.5e-6
-        # and every other object below 3.5e-6).  Objects of 64 and more
-        # observations meet the 1e-5 relative bound.
+        # finest grid the VALU tile's error reaches ~1.2e-5 relative and
+        # ~8e-6 P_max (a CPU emulation with correctly rounded sin/cos puts
+        # that object at 6.5e-6 and every other object below 3.5e-6).  Objects
+        # of 64 and more observations meet the 1e-5 relative bound.
         big = np.array(counts) >= 64
         for name, sel, rtol, atol in (("N >= 64", big, 1e-5, 5e-6), ("N = 5", ok & ~big, 3e-5, 1.5e-5)):
             print(name, end=": ")
             assert_close(pg[sel], ref[0][sel], rtol, 1e-3, atol)
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_split32_standard_valu_and_matrix_core_tiles(golden, dtype):
+    """The standard split32 sweep has two kernels: the matrix-core tile
+    (64 threads x 1024 bins, the default) and the VALU tile (256 x 8).  Both
+    against the FP64 oracle on the synthetic batch, the JD-scale ZTF object
+    and objects around the staging and fold lengths, and against each other."""
+    L = _native.core()
+    o, t, y = golden["syn_std_obj"], golden["syn_std_t"], golden["syn_std_y"]
+    t, y = t.astype(dtype).astype(F64), y.astype(dtype).astype(F64)
+    ref = oracle.batch(o, t, y, None, synth.OMEGA_MIN, synth.OMEGA_MAX, 20000, False, F64, nthreads=8)
+    zo, zt, zy, _ = load_data("243_normalized_ztf_filter2.txt")  # t ~ 2.4587e6 (JD)
+    zref = oracle.single(zt, zy, None, 3.142, 150.796, 20000, False, F64, nthreads=8)
+    rng = np.random.default_rng(29)
+    counts = [16, 17, 63, 255, 256, 257, 511, 700]   # fold length 256, blocks of 16
+    ids = np.repeat(np.arange(len(counts), dtype=np.uint32), counts)
+    rt = np.concatenate([np.sort(rng.uniform(0, 1800, n)) for n in counts])
+    ry = np.sin(2.1 * rt) + rng.normal(0, 0.5, rt.size)
+    rref = oracle.batch(ids, rt, ry, None, synth.OMEGA_MIN, synth.OMEGA_MAX, 9000, False, F64, nthreads=8)
+    out = {}
+    try:
+        for tile in ((256, 8), (64, 16)):
+            assert L.lsamd_set_tile(_native.FORM_SPLIT32, _native.STANDARD, *tile) == _native.OK
+            r = _native.batch_host(o, t, y, None, synth.OMEGA_MIN, synth.OMEGA_MAX, 20000,
+                                   _native.STANDARD, dtype, _native.FORM_SPLIT32)
+            print(tile, end=": ")
+            assert_close(r["pgram"], ref[0], 1e-5, 1e-3, 2e-6)
+            assert argmax_agrees(r["pgram"], ref[0], 1e-5) == 0
+            z = _native.batch_host(zo, zt, zy, None, 3.142, 150.796, 20000, _native.STANDARD, F64,
+                                   _native.FORM_SPLIT32)
+            assert_close(z["pgram"], zref[0], 1e-5, 1e-3, 2e-6)
+            g = _native.batch_host(ids, rt, ry, None, synth.OMEGA_MIN, synth.OMEGA_MAX, 9000,
+                                   _native.STANDARD, F64, _native.FORM_SPLIT32)
+            assert_close(g["pgram"], rref[0], 1e-5, 1e-3, 2e-6)
+            out[tile] = r["pgram"]
+    finally:
+        L.lsamd_set_tile(_native.FORM_SPLIT32, _native.STANDARD, 0, 0)
+    assert_close(out[(64, 16)], out[(256, 8)], 2e-5, 1e-3, 4e-6)
 
 
 def test_many_tiny_objects():
