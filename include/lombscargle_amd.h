@@ -76,11 +76,13 @@ enum lsamd_form {
      * power: within ~2e-6 relative of FP64 on the bins that carry power
      * (P >= 1e-3 Pmax), including aliasing frequencies */
     LSAMD_FORM_SPLIT = 1,
-    /* as SPLIT but with FP32 steps across frequencies (rotation, and for the
-     * standard variant the three-term recurrence where |sin D| >= 0.25) and
-     * FP32 sums over 32 (standard) or 64 (generalized) observations folded
-     * into FP64: the fastest form;
-     * within 1e-5 of FP64 on well-conditioned data (7.5e-6 on a 1000-object
+    /* as SPLIT but with FP32 arithmetic across frequencies and observations:
+     * standard variant (default tile) on the FP16 matrix cores, the sums of
+     * 32 x 32 bins as row x column factor products with head/tail splitting,
+     * FP32 accumulation folded into FP64 every 256 observations; generalized
+     * variant (and the standard VALU tiles) FP32 rotation / recurrence and FP32
+     * sums over 32 or 64 observations folded into FP64: the fastest form;
+     * within 1e-5 of FP64 on well-conditioned data (6.4e-6 on a 1000-object
      * standard batch), up to ~3e-5 on generalized data and ~1e-4 at strong
      * aliases */
     LSAMD_FORM_SPLIT32 = 2
@@ -284,7 +286,9 @@ double lsamd_last_sweep_kernel_ms(void);
 
 /* Tile shape of the sweep kernel: threads per workgroup and frequencies per
  * thread, among the shapes compiled for the form and variant (64..512
- * threads, 1..16 frequencies).  Zero restores the built-in choice.  Returns
+ * threads, 1..16 frequencies).  SPLIT32 standard 64 x 16 (the built-in
+ * choice) is the matrix-core kernel; 256 x 8 the fastest VALU one.  Zero
+ * restores the built-in choice.  Returns
  * LSAMD_ERR_INVALID for shapes not compiled.  Process-wide setting (as is
  * the choice lsamd_autotune makes); set it before sweeping from several
  * threads, not while they run. */
