@@ -97,8 +97,14 @@ def test_observation_slices_workspace():
     sliced = L.lsamd_sweep_workspace_size(byref(a))
     a.obs_slices = 1
     whole = L.lsamd_sweep_workspace_size(byref(a))
-    # 49 chunks of 256 x 8 bins, 14 slices of <= 256 observations, 4 sums
-    assert sliced - whole >= 49 * 14 * 4 * 8 * 256 * 8
+    # chunks of T x M bins, slices of <= T observations (one LDS stage), 4 sums
+    from ctypes import c_int
+    T, M = c_int(), c_int()
+    L.lsamd_get_tile(a.form, a.variant, a.dtype, byref(T), byref(M))
+    chunks = -(-100000 // (T.value * M.value))
+    slices = min(-(-2048 // chunks), -(-3555 // T.value))
+    assert slices >= 2
+    assert sliced - whole >= chunks * slices * 4 * M.value * T.value * 8
     a.obs_slices, a.num_objects = 0, 2
     assert L.lsamd_sweep_workspace_size(byref(a)) < sliced
     a.form, a.num_objects = _native.FORM_PLAIN, 1
@@ -107,6 +113,40 @@ def test_observation_slices_workspace():
     a.offsets = a.t = a.y = 1  # never dereferenced: rejected before any launch
     assert L.lsamd_sweep(byref(a), c_void_p(4096), 1 << 30, None) == _native.ERR_INVALID
     assert b"obs_slices" in L.lsamd_last_error()
+
+
+def test_matrix_core_tile_workspace():
+    """The split32 standard matrix-core tile (64 x 16) reserves its column-
+    factor fragments: 12 fragments x 64 lanes x 16 B per 16-observation block,
+    in slots floor(beg/16) + k .. + ceil(N/16) per object, which never overlap
+    and end below n/16 + K + 1."""
+    from ctypes import c_int
+    L = _native.core()
+    a = _native.SweepArgs()
+    a.variant, a.dtype, a.form = _native.STANDARD, _native.F64, _native.FORM_SPLIT32
+    a.num_objects, a.num_freqs, a.num_obs, a.top_k, a.obs_slices = 1000, 1000000, 163652, 1, 1
+    a.min_freq, a.max_freq = 1.005, 150.796
+    T, M = c_int(), c_int()
+    L.lsamd_get_tile(a.form, a.variant, a.dtype, byref(T), byref(M))
+    assert (T.value, M.value) == (64, 16)   # the built-in split32 standard tile
+    try:
+        mfma = L.lsamd_sweep_workspace_size(byref(a))
+        assert L.lsamd_set_tile(a.form, a.variant, 256, 8) == _native.OK
+        valu = L.lsamd_sweep_workspace_size(byref(a))
+    finally:
+        L.lsamd_set_tile(a.form, a.variant, 0, 0)
+    colfac = 12 * 64 * 16 * (163652 // 16 + 1000 + 1)
+    assert mfma - valu >= colfac   # (its 1024-bin chunks also need more peak partials)
+    assert L.lsamd_set_tile(a.form, _native.GENERALIZED, 64, 16) == _native.ERR_INVALID
+    # the slot rule on random object sizes (including empty and 1-observation objects)
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        sizes = rng.integers(0, 90, rng.integers(1, 60))
+        beg = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+        first = beg // 16 + np.arange(sizes.size)
+        last = first + (sizes + 15) // 16          # one past the object's last slot
+        assert np.all(last[:-1] <= first[1:])
+        assert last.max() <= sizes.sum() // 16 + sizes.size + 1
 
 
 def test_point_argument_checks():
