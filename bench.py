@@ -639,7 +639,7 @@ def main():
         "metric": "freq*obs terms/s and objects/s, FP32-split vs FP64 at <=1e-5 power error",
         "value": value, "unit": "terms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(r["ms"], 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f16x2+f32+f64" if mfma else "f32+f64", "data": "synthetic",
         "config": {"workload": "standard L-S batch: 1000 objects, log-normal N (median 140, sigma 0.6, "
                                "20..1000), 1e6 angular freqs 1.005..150.796 rad/d, FP64 inputs in HBM, "
                                "f64 periodogram written + fused argmax",
