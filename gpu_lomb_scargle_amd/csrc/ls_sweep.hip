@@ -5,9 +5,11 @@
 //
 // Work decomposition: work items (object, frequency chunk), one workgroup
 // each, or taken in groups of consecutive chunks.  A chunk is T*M
-// consecutive frequency bins; thread tid owns bins chunk*T*M + m*T + tid
-// (m < M).  The object's observations are staged through LDS T at a time and
-// read back as wave-wide broadcasts; all sums live in registers.
+// consecutive frequency bins; in the VALU kernels thread tid owns bins
+// chunk*T*M + m*T + tid (m < M).  The object's observations are staged
+// through LDS T at a time and read back as wave-wide broadcasts; all sums
+// live in registers.  The matrix-core tile (64 x 16) maps a chunk to a
+// 32 x 32 block of bins instead (see k_sweep_mfma).
 // Per-object normalisation and the argmax/top-3 peak search are fused: every
 // workgroup writes its chunk's best bin (or best three) and a small
 // reduction kernel merges the chunks of each object.
@@ -21,8 +23,11 @@
 //             hardware sin/cos seed corrected in FP64, then the thread's M
 //             bins by an FP64 rotation and three-term recurrence (step T
 //             bins) with FP64 sums.
-//   split32 - the same seed in FP32, an FP32 rotation across the bins, and
-//             FP32 partial sums over 32-64 observations folded into FP64.
+//   split32 - the same seed in FP32; the VALU tiles step across the bins in
+//             FP32 (rotation, recurrence) with FP32 partial sums over 32-64
+//             observations folded into FP64; the standard variant's default
+//             tile forms the sums on the FP16 matrix cores as row x column
+//             factor products (head/tail split), folded into FP64.
 // Both split forms use the tau-free FP64 power
 //     standard:    P = (SS XC^2 - 2 CS XC XS + CC XS^2) / (2 (CC SS - CS^2))
 //     generalized: P = (SS YC^2 - 2 CS YC YS + CC YS^2) / ((CC SS - CS^2) YY)

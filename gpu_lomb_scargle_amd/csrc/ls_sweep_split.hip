@@ -1,6 +1,7 @@
 // Split-form sweep kernels (split: FP64 rotation and sums; split32: FP32
-// rotation and FP32 partial sums folded into FP64), the observation-slice
-// combiner, and their tile registry entries.
+// rotation and FP32 partial sums folded into FP64, and for the standard
+// variant the matrix-core tile), the observation-slice combiner, and their
+// tile registry entries.
 
 #include "ls_device.h"
 
