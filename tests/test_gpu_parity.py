@@ -471,6 +471,25 @@ def test_split32_standard_valu_and_matrix_core_tiles(golden, dtype):
     assert_close(out[(64, 16)], out[(256, 8)], 2e-5, 1e-3, 4e-6)
 
 
+@pytest.mark.parametrize("scale", [1e-6, 1.0, 3e5])
+def test_split32_matrix_core_tile_y_range(scale):
+    """The matrix-core tile scales each object's y by a power of two into the
+    FP16 range and back: magnitudes from 1e-6 to 3e5 (and an all-zero object,
+    whose power is 0) give the same relative accuracy."""
+    b = synth.lognormal_batch(24, 140, 0.6, 20, 1000, 1800, False, 77)
+    y = b["y"] * scale
+    off = np.concatenate([[0], np.cumsum(b["counts"])])
+    y[off[3]:off[4]] = 0.0
+    ref = oracle.batch(b["obj_id"], b["t"], y, None, synth.OMEGA_MIN, synth.OMEGA_MAX, 20000, False, F64,
+                       normalize=False, nthreads=8)[0]
+    r = _native.batch_host(b["obj_id"], b["t"], y, None, synth.OMEGA_MIN, synth.OMEGA_MAX, 20000,
+                           _native.STANDARD, F64, _native.FORM_SPLIT32, normalize=False)
+    pg = r["pgram"]
+    assert np.all(pg[3] == 0.0)
+    keep = np.arange(24) != 3
+    assert_close(pg[keep], ref[keep], 1e-5, 1e-3, 2e-6)
+
+
 def test_many_tiny_objects():
     b = synth.lognormal_batch(3000, 8, 0.5, 5, 40, 1800, False, 5)
     ref = oracle.batch(b["obj_id"], b["t"], b["y"], None, synth.OMEGA_MIN, synth.OMEGA_MAX, 300,
