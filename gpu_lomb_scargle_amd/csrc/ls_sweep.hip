@@ -25,9 +25,9 @@
 //             bins) with FP64 sums.
 //   split32 - the same seed in FP32; the VALU tiles step across the bins in
 //             FP32 (rotation, recurrence) with FP32 partial sums over 32-64
-//             observations folded into FP64; the standard variant's default
-//             tile forms the sums on the FP16 matrix cores as row x column
-//             factor products (head/tail split), folded into FP64.
+//             observations folded into FP64; the default tiles form the
+//             sums on the FP16 matrix cores as row x column factor products
+//             (head/tail split), folded into FP64.
 // Both split forms use the tau-free FP64 power
 //     standard:    P = (SS XC^2 - 2 CS XC XS + CC XS^2) / (2 (CC SS - CS^2))
 //     generalized: P = (SS YC^2 - 2 CS YC YS + CC YS^2) / ((CC SS - CS^2) YY)
@@ -327,11 +327,11 @@ int g_tile_M[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
 
 // Built-in shapes: the fastest of the tile sweep on the 1000-object x 1e6
 // batches (MI355X): split 256x16 (standard) / 256x8 (generalized); split32
-// 64x16, the matrix-core tile (standard; 256x8 is the fastest VALU tile) /
-// 256x4 (generalized).  lsamd_autotune re-picks per workload.
+// 64x16, the matrix-core tiles (the fastest VALU tiles: 256x8 standard,
+// 256x4 generalized).  lsamd_autotune re-picks per workload.
 Tile default_tile(int form, int variant) {
     if (form == LSAMD_FORM_SPLIT) return variant ? Tile{256, 8} : Tile{256, 16};
-    if (form == LSAMD_FORM_SPLIT32) return variant ? Tile{256, 4} : Tile{64, 16};
+    if (form == LSAMD_FORM_SPLIT32) return Tile{64, 16};
     return Tile{256, 1};
 }
 

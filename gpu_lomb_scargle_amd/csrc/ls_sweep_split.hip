@@ -1,7 +1,6 @@
 // Split-form sweep kernels (split: FP64 rotation and sums; split32: FP32
-// rotation and FP32 partial sums folded into FP64, and for the standard
-// variant the matrix-core tile), the observation-slice combiner, and their
-// tile registry entries.
+// rotation and FP32 partial sums folded into FP64, and the matrix-core
+// tiles), the observation-slice combiner, and their tile registry entries.
 
 #include "ls_device.h"
 
@@ -479,7 +478,8 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
 
 
 // ---------------------------------------------------------------------------
-// Matrix-core tile of split32, standard variant.  A work item is a block of
+// Matrix-core tile of split32 (standard variant here, generalized in
+// k_sweep_mfma_gen).  A work item is a block of
 // 32 x 32 bins, bin = chunk * 1024 + 32 a + b.  Its sums are complex products
 // of a row factor (the frequency f_a of column 0) and a column factor (b df):
 //   Z1[a][b] = sum_j (y_j e^{2 pi i f_a t_j}) e^{2 pi i b df t_j} = XC + i XS
@@ -778,6 +778,239 @@ __global__ __launch_bounds__(64) void k_sweep_mfma(SweepArgs a) {
     }
 }
 
+
+// Generalized variant of the matrix-core tile.  With q = sqrt(w / W) (the
+// records' normalised weights) and y_c = y - ybar, three complex sums per bin:
+//   Zw = sum q^2 e^{i phi},  Zy = sum q (q y_c) e^{i phi},  Z2 = sum q^2 e^{2 i phi}
+// give the seven weighted sums of the generalized power: C, S = Zw;
+// YC, YS = Zy; CC, SS = (Wf +- Re Z2) / 2; CS = Im Z2 / 2, with Wf = sum q^2.
+// 36 MFMAs and 12 accumulators per 16 observations; the FP64 fold holds six
+// sums (48 KiB of LDS per workgroup).
+template <typename OUT>
+__global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
+    constexpr int T = 64, M = 16;
+    constexpr uint32_t kFoldObs = 256;
+    __shared__ double s_t[T];
+    __shared__ float s_q[T], s_y[T];
+    __shared__ double s_acc[6][16][T];  // FP64 Re/Im of Zw, Zy, Z2 of a lane's 16 bins
+    __shared__ double s_pp[1][kPeaks];
+    __shared__ uint32_t s_pi[1][kPeaks];
+    __shared__ uint64_t s_item;
+    const uint32_t l = threadIdx.x, rc = l & 31u, kh = l >> 5;
+    constexpr double kTail = 1.0 / 2048.0;
+    constexpr double kUnscale = 1.0 / ((double)kFacScale * (double)kFacScale);
+    for (uint64_t item = first_item(a, &s_item); item < a.total_items;
+         item = next_item(a, &s_item, item)) {
+        uint64_t oc = item;
+        uint32_t slice = 0, obj, chunk;
+        if (a.slices > 1) {
+            oc = item / a.slices;
+            slice = (uint32_t)(item - oc * a.slices);
+        }
+        item_coords(a, oc, obj, chunk);
+        const uint64_t obeg = a.offsets[obj], oend = a.offsets[obj + 1];
+        uint64_t beg = obeg, end = oend;
+        if (a.slices > 1) {
+            const uint64_t len = (((oend - obeg) + a.slices - 1) / a.slices + 15u) & ~(uint64_t)15u;
+            const uint64_t b = obeg + (uint64_t)slice * len;
+            beg = b < oend ? b : oend;
+            end = (beg + len) < oend ? beg + len : oend;
+        }
+        const float ys = a.obj_ys[obj];
+        const uint32_t kbase = chunk * (uint32_t)(T * M);
+        const double fa = a.fmin_rev + (double)(kbase + 32u * rc) * a.df_rev;
+        const uint4 *cf = a.colfac +
+                          (colfac_slot(obeg, obj) + ((beg - obeg) >> 4)) * (uint64_t)(kColfacFrags * 64) + l;
+        f32x16_t zw[4] = {}, zy[4] = {}, z2[4] = {};  // {Re hh, Re x, Im hh, Im x}
+        double wq = 0.0;                             // this lane's share of sum q^2 (slices)
+        bool folded = false;
+        auto fold = [&]() {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                double p[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) p[q] = folded ? s_acc[q][v][l] : 0.0;
+                s_acc[0][v][l] = p[0] + (double)zw[0][v] + kTail * (double)zw[1][v];
+                s_acc[1][v][l] = p[1] + (double)zw[2][v] + kTail * (double)zw[3][v];
+                s_acc[2][v][l] = p[2] + (double)zy[0][v] + kTail * (double)zy[1][v];
+                s_acc[3][v][l] = p[3] + (double)zy[2][v] + kTail * (double)zy[3][v];
+                s_acc[4][v][l] = p[4] + (double)z2[0][v] + kTail * (double)z2[1][v];
+                s_acc[5][v][l] = p[5] + (double)z2[2][v] + kTail * (double)z2[3][v];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                zw[i] = f32x16_t{};
+                zy[i] = f32x16_t{};
+                z2[i] = f32x16_t{};
+            }
+            folded = true;
+        };
+        for (uint64_t g0 = beg; g0 < end; g0 += kFoldObs) {
+        const uint64_t gend = (end - g0) < kFoldObs ? end : g0 + kFoldObs;
+        for (uint64_t b0 = g0; b0 < gend; b0 += T) {
+            const uint32_t cnt = (uint32_t)((gend - b0) < (uint64_t)T ? (gend - b0) : (uint64_t)T);
+            __syncthreads();  // also orders the previous item's peak merge before this one's
+            if (l < cnt) {
+                const float4 r = a.rec_a[b0 + l];
+                s_t[l] = a.rec_t[b0 + l];
+                s_q[l] = r.y;
+                s_y[l] = r.x * (ys * kFacScale);
+                wq += (double)(r.y * r.y);
+            }
+            __syncthreads();
+            for (uint32_t j0 = 0; j0 < cnt; j0 += 16, cf += kColfacFrags * 64) {
+                half8_t col[kColfacFrags];
+#pragma unroll
+                for (int i = 0; i < kColfacFrags; ++i) {
+                    const uint4 u = cf[i * 64];
+                    __builtin_memcpy(&col[i], &u, sizeof(u));
+                }
+                half8_t wr_h, wr_l, wi_h, wi_l, yr_h, yr_l, yi_h, yi_l, qr_h, qr_l, qi_h, qi_l;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const uint32_t j = j0 + 8u * kh + (uint32_t)v;  // < T: j0 <= T - 16
+                    const bool ok = j < cnt;
+                    const double tj = s_t[j];
+                    const float qj = ok ? s_q[j] : 0.0f, yj = ok ? s_y[j] : 0.0f;
+                    double ph = fa * tj;
+                    ph -= __builtin_rint(ph);
+                    float c = __builtin_amdgcn_cosf((float)ph);
+                    float s = __builtin_amdgcn_sinf((float)ph);
+                    c = ok ? c : 0.0f;
+                    s = ok ? s : 0.0f;
+                    const float q2 = qj * qj * kFacScale, qy = qj * yj;
+                    _Float16 h, t;
+                    split16(q2 * c, h, t);
+                    wr_h[v] = h;
+                    wr_l[v] = t;
+                    split16(q2 * s, h, t);
+                    wi_h[v] = h;
+                    wi_l[v] = t;
+                    split16(qy * c, h, t);
+                    yr_h[v] = h;
+                    yr_l[v] = t;
+                    split16(qy * s, h, t);
+                    yi_h[v] = h;
+                    yi_l[v] = t;
+                    const float cc = c * c, ss = s * s, g = q2 * (2.0f - (cc + ss));  // q^2 u^2/|u|^2
+                    split16((cc - ss) * g, h, t);
+                    qr_h[v] = h;
+                    qr_l[v] = t;
+                    split16(2.0f * c * s * g, h, t);
+                    qi_h[v] = h;
+                    qi_l[v] = t;
+                }
+                // Re Z = p_r f_r - p_i f_i,  Im Z = p_r f_i + p_i f_r; column
+                // fragments f = v (col 0..5) for Zw, Zy and w = v^2 (col 6..11) for Z2
+                LS_MFMA(zw[0], wr_h, col[0]);
+                LS_MFMA(zw[1], wr_h, col[1]);
+                LS_MFMA(zw[1], wr_l, col[0]);
+                LS_MFMA(zw[0], wi_h, col[4]);
+                LS_MFMA(zw[1], wi_h, col[5]);
+                LS_MFMA(zw[1], wi_l, col[4]);
+                LS_MFMA(zw[2], wr_h, col[2]);
+                LS_MFMA(zw[3], wr_h, col[3]);
+                LS_MFMA(zw[3], wr_l, col[2]);
+                LS_MFMA(zw[2], wi_h, col[0]);
+                LS_MFMA(zw[3], wi_h, col[1]);
+                LS_MFMA(zw[3], wi_l, col[0]);
+                LS_MFMA(zy[0], yr_h, col[0]);
+                LS_MFMA(zy[1], yr_h, col[1]);
+                LS_MFMA(zy[1], yr_l, col[0]);
+                LS_MFMA(zy[0], yi_h, col[4]);
+                LS_MFMA(zy[1], yi_h, col[5]);
+                LS_MFMA(zy[1], yi_l, col[4]);
+                LS_MFMA(zy[2], yr_h, col[2]);
+                LS_MFMA(zy[3], yr_h, col[3]);
+                LS_MFMA(zy[3], yr_l, col[2]);
+                LS_MFMA(zy[2], yi_h, col[0]);
+                LS_MFMA(zy[3], yi_h, col[1]);
+                LS_MFMA(zy[3], yi_l, col[0]);
+                LS_MFMA(z2[0], qr_h, col[6]);
+                LS_MFMA(z2[1], qr_h, col[7]);
+                LS_MFMA(z2[1], qr_l, col[6]);
+                LS_MFMA(z2[0], qi_h, col[10]);
+                LS_MFMA(z2[1], qi_h, col[11]);
+                LS_MFMA(z2[1], qi_l, col[10]);
+                LS_MFMA(z2[2], qr_h, col[8]);
+                LS_MFMA(z2[3], qr_h, col[9]);
+                LS_MFMA(z2[3], qr_l, col[8]);
+                LS_MFMA(z2[2], qi_h, col[6]);
+                LS_MFMA(z2[3], qi_h, col[7]);
+                LS_MFMA(z2[3], qi_l, col[6]);
+            }
+        }
+        if (gend < end) fold();
+        }
+        // the item's FP64 sums of result entry v: Re/Im Zw, Zy (y unscaled), Z2
+        auto sums = [&](int v, double (&z)[6]) {
+            z[0] = (double)zw[0][v] + kTail * (double)zw[1][v];
+            z[1] = (double)zw[2][v] + kTail * (double)zw[3][v];
+            z[2] = (double)zy[0][v] + kTail * (double)zy[1][v];
+            z[3] = (double)zy[2][v] + kTail * (double)zy[3][v];
+            z[4] = (double)z2[0][v] + kTail * (double)z2[1][v];
+            z[5] = (double)z2[2][v] + kTail * (double)z2[3][v];
+            if (folded) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) z[q] += s_acc[q][v][l];
+            }
+            const double iys = 1.0 / (double)ys;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) z[q] *= kUnscale * (q == 2 || q == 3 ? iys : 1.0);
+        };
+        if (a.slices > 1) {
+            // the slice's seven FP64 sums in the layout k_combine_slices reads,
+            // with the slice's weight sum for CC and SS
+            double wsum = wq;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) wsum += __shfl_xor(wsum, off, 64);
+            double *dst = a.part_sums + (((uint64_t)obj * a.chunks + chunk) * a.slices + slice) *
+                                            (uint64_t)(7 * M * T);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t row = 8u * (uint32_t)(v / 4) + 4u * kh + (uint32_t)(v % 4);
+                const uint32_t bl = 32u * row + rc, m = bl / T, tt = bl % T;
+                double z[6];
+                sums(v, z);
+                dst[(0 * M + m) * T + tt] = z[0];
+                dst[(1 * M + m) * T + tt] = z[1];
+                dst[(2 * M + m) * T + tt] = z[2];
+                dst[(3 * M + m) * T + tt] = z[3];
+                dst[(4 * M + m) * T + tt] = 0.5 * (wsum + z[4]);
+                dst[(5 * M + m) * T + tt] = 0.5 * (wsum - z[4]);
+                dst[(6 * M + m) * T + tt] = 0.5 * z[5];
+            }
+            continue;
+        }
+        const double scale = a.obj_scale[obj];
+        const double Wf = a.obj_gen[kGen * obj + 0], Yb = a.obj_gen[kGen * obj + 1];
+        const double YY = a.obj_gen[kGen * obj + 2], iW = 1.0 / Wf;
+        OUT *pg = static_cast<OUT *>(a.pgram);
+        Peaks L;
+        peaks_init(L);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const uint32_t row = 8u * (uint32_t)(v / 4) + 4u * kh + (uint32_t)(v % 4);
+            const uint32_t kf = kbase + 32u * row + rc;  // ascending in v
+            double z[6];
+            sums(v, z);
+            const double C = z[0] * iW, S = z[1] * iW;
+            const double YC = z[2] * iW - Yb * C, YS = z[3] * iW - Yb * S;
+            const double CC = 0.5 * (Wf + z[4]) * iW - C * C;
+            const double SS = 0.5 * (Wf - z[4]) * iW - S * S;
+            const double CS = 0.5 * z[5] * iW - C * S;
+            const double D = CC * SS - CS * CS;
+            const double P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
+            const OUT pw = (OUT)(P * scale);
+            if (kf < a.nf) {
+                if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
+                peaks_add(L, a.top_k, (double)pw, kf);
+            }
+        }
+        if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+    }
+}
+
 }  // namespace
 
 #define LS_SPLIT32(GEN, OUTT, DT, TT, MM)                                      \
@@ -793,6 +1026,10 @@ const TileEntry kSplitTiles[] = {
      k_combine_slices<true, false, float, 64, 16>, k_colfac},
     {LSAMD_FORM_SPLIT32, 0, LSAMD_F64, 64, 16, k_sweep_mfma<double>,
      k_combine_slices<true, false, double, 64, 16>, k_colfac},
+    {LSAMD_FORM_SPLIT32, 1, LSAMD_F32, 64, 16, k_sweep_mfma_gen<float>,
+     k_combine_slices<true, true, float, 64, 16>, k_colfac},
+    {LSAMD_FORM_SPLIT32, 1, LSAMD_F64, 64, 16, k_sweep_mfma_gen<double>,
+     k_combine_slices<true, true, double, 64, 16>, k_colfac},
     // split form (FP64 rotation and sums), both output types
     LS_SPLIT(0, float, LSAMD_F32, 256, 8), LS_SPLIT(0, double, LSAMD_F64, 256, 8),
     LS_SPLIT(0, float, LSAMD_F32, 256, 4), LS_SPLIT(0, double, LSAMD_F64, 256, 4),

@@ -77,11 +77,11 @@ enum lsamd_form {
      * (P >= 1e-3 Pmax), including aliasing frequencies */
     LSAMD_FORM_SPLIT = 1,
     /* as SPLIT but with FP32 arithmetic across frequencies and observations:
-     * standard variant (default tile) on the FP16 matrix cores, the sums of
+     * the default tile (both variants) on the FP16 matrix cores, the sums of
      * 32 x 32 bins as row x column factor products with head/tail splitting,
-     * FP32 accumulation folded into FP64 every 256 observations; generalized
-     * variant (and the standard VALU tiles) FP32 rotation / recurrence and FP32
-     * sums over 32 or 64 observations folded into FP64: the fastest form;
+     * FP32 accumulation folded into FP64 every 256 observations; the VALU
+     * tiles FP32 rotation / recurrence and FP32 sums over 32 or 64
+     * observations folded into FP64: the fastest form;
      * within 1e-5 of FP64 on well-conditioned data (6.4e-6 on a 1000-object
      * standard batch), up to ~3e-5 on generalized data and ~1e-4 at strong
      * aliases */
@@ -286,9 +286,9 @@ double lsamd_last_sweep_kernel_ms(void);
 
 /* Tile shape of the sweep kernel: threads per workgroup and frequencies per
  * thread, among the shapes compiled for the form and variant (64..512
- * threads, 1..16 frequencies).  SPLIT32 standard 64 x 16 (the built-in
- * choice) is the matrix-core kernel; 256 x 8 the fastest VALU one.  Zero
- * restores the built-in choice.  Returns
+ * threads, 1..16 frequencies).  SPLIT32 64 x 16 (the built-in choice) is the
+ * matrix-core kernel; 256 x 8 (standard) and 256 x 4 (generalized) the
+ * fastest VALU ones.  Zero restores the built-in choice.  Returns
  * LSAMD_ERR_INVALID for shapes not compiled.  Process-wide setting (as is
  * the choice lsamd_autotune makes); set it before sweeping from several
  * threads, not while they run. */

@@ -471,6 +471,30 @@ def test_split32_standard_valu_and_matrix_core_tiles(golden, dtype):
     assert_close(out[(64, 16)], out[(256, 8)], 2e-5, 1e-3, 4e-6)
 
 
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_split32_generalized_valu_and_matrix_core_tiles(golden, dtype):
+    """The generalized split32 sweep's two kernels, the matrix-core tile
+    (default) and the VALU tile (256 x 4), against the FP64 oracle on the
+    synthetic generalized batch and against each other."""
+    L = _native.core()
+    o, t, y, dy = golden["syn_gen_obj"], golden["syn_gen_t"], golden["syn_gen_y"], golden["syn_gen_dy"]
+    t, y, dy = t.astype(dtype).astype(F64), y.astype(dtype).astype(F64), dy.astype(dtype).astype(F64)
+    ref = oracle.batch(o, t, y, dy, synth.OMEGA_MIN, synth.OMEGA_MAX, 20000, True, F64, nthreads=8)
+    out = {}
+    try:
+        for tile in ((256, 4), (64, 16)):
+            assert L.lsamd_set_tile(_native.FORM_SPLIT32, _native.GENERALIZED, *tile) == _native.OK
+            r = _native.batch_host(o, t, y, dy, synth.OMEGA_MIN, synth.OMEGA_MAX, 20000,
+                                   _native.GENERALIZED, dtype, _native.FORM_SPLIT32)
+            print(tile, end=": ")
+            assert_close(r["pgram"], ref[0], 1e-5, 1e-3, 2e-6)
+            assert argmax_agrees(r["pgram"], ref[0], 1e-5) == 0
+            out[tile] = r["pgram"]
+    finally:
+        L.lsamd_set_tile(_native.FORM_SPLIT32, _native.GENERALIZED, 0, 0)
+    assert_close(out[(64, 16)], out[(256, 4)], 2e-5, 1e-3, 4e-6)
+
+
 @pytest.mark.parametrize("scale", [1e-6, 1.0, 3e5])
 def test_split32_matrix_core_tile_y_range(scale):
     """The matrix-core tile scales each object's y by a power of two into the
