@@ -630,7 +630,7 @@ def main():
 
     forms = {"split32": "FP64 phase reduction, FP32 hardware sin/cos seeds; matrix-core tile: per 32x32 bins the "
                         "sums as complex products of a row and a column factor on FP16 matrix cores (head/tail "
-                        "split, FP32 accumulation folded into FP64 every 256 observations), FP64 power",
+                        "split, FP32 accumulation folded every 256 observations), FP64 power",
              "split32_valu": "FP64 phase reduction, FP32 hardware sin/cos seeds, FP32 rotation or three-term "
                              "recurrence across bins, 32-observation FP32 partial sums folded into FP64, FP64 power",
              "split": "FP64 phase reduction, FP32 hardware sin/cos seeds, FP64 rotation recurrence and sums, "

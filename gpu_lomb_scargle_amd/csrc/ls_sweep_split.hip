@@ -487,11 +487,13 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
 // with CC = (N + C2) / 2, CS = S2 / 2, SS = N - CC.  Each complex sum is four
 // real products over blocks of 16 observations, formed by
 // v_mfma_f32_32x32x16_f16 with FP32 accumulation.  Every factor, scaled by
-// 2^10, is split into an FP16 head h and a tail l = fp16((x - h) 2^11); head x
-// head and the two cross products carry ~22 bits (the dropped tail x tail
-// term is 2^-22), in two accumulators combined as (hh + 2^-11 x) 2^-20.  The
-// 2^10 scale keeps FP16 subnormal heads (which the VALU conversions and the
-// matrix core do not treat alike) to |x| < 2^-24.  Phases are reduced in FP64
+// 2^10, is split into an FP16 head h and a tail l = fp16(x - h); head x head
+// and the two cross products carry ~22 bits (the dropped tail x tail term is
+// 2^-22 of the product); each complex sum's head x head and cross products
+// accumulate in FP32 (apart, or in one sum for Z2 of the standard variant),
+// unscaled by 2^-20 in FP64.  The 2^10 scale keeps FP16 subnormal heads (which the VALU
+// conversions and the matrix core do not treat alike) to |x| < 2^-24; a
+// subnormal tail costs at most 2^-24 absolute.  Phases are reduced in FP64
 // and evaluated by the hardware sin/cos, as in the VALU split32 sweep.  The
 // factors of Z2 are divided by their squared modulus (hardware sin/cos leave
 // |z|^2 - 1 ~ 1e-7, and CC = (N + C2) / 2 assumes unit modulus).
@@ -509,7 +511,7 @@ constexpr float kFacScale = 1024.0f;  // 2^10 on each factor
 
 __device__ __forceinline__ void split16(float x, _Float16 &h, _Float16 &l) {
     h = (_Float16)(fabsf(x) < 0x1p-14f ? 0.0f : x);  // no FP16 subnormal heads
-    l = (_Float16)((x - (float)h) * 2048.0f);
+    l = (_Float16)(x - (float)h);                     // x - h is exact in FP32
 }
 
 // First column-factor slot of an object: the slots of object k are
@@ -581,18 +583,22 @@ __global__ __launch_bounds__(64) void k_colfac(SweepArgs a) {
 
 #define LS_MFMA(ACC, A, B) ACC = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B, ACC, 0, 0, 0)
 
+// Two waves per SIMD: six FP32 accumulators (96 registers; Z1 keeps its
+// cross products apart, where merging them doubles the rounding error of the
+// power, while Z2 takes all three products in one) leave room for a second
+// workgroup's registers, and the fold of the 256-observation partials stays
+// in FP32 (16 KiB of LDS, 8 workgroups per CU).
 template <typename OUT>
-__global__ __launch_bounds__(64) void k_sweep_mfma(SweepArgs a) {
+__global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
     constexpr int T = 64, M = 16;  // 64 threads, 1024 bins per item
-    constexpr uint32_t kFoldObs = 256;  // FP32 accumulation length
+    constexpr uint32_t kFoldObs = 256;  // MFMA accumulation length
     __shared__ double s_t[T];
     __shared__ float s_y[T];
-    __shared__ double s_acc[4][16][T];  // FP64 sums XC, XS, C2, S2 of a lane's 16 bins
+    __shared__ float s_acc[4][16][T];  // sums of the 256-observation partials XC, XS, C2, S2
     __shared__ double s_pp[1][kPeaks];
     __shared__ uint32_t s_pi[1][kPeaks];
     __shared__ uint64_t s_item;
     const uint32_t l = threadIdx.x, rc = l & 31u, kh = l >> 5;
-    constexpr double kTail = 1.0 / 2048.0;
     constexpr double kUnscale = 1.0 / ((double)kFacScale * (double)kFacScale);
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
          item = next_item(a, &s_item, item)) {
@@ -617,29 +623,25 @@ __global__ __launch_bounds__(64) void k_sweep_mfma(SweepArgs a) {
         const double fa = a.fmin_rev + (double)(kbase + 32u * rc) * a.df_rev;  // row rc, column 0
         const uint4 *cf = a.colfac +
                           (colfac_slot(obeg, obj) + ((beg - obeg) >> 4)) * (uint64_t)(kColfacFrags * 64) + l;
-        f32x16_t z1r_hh = {}, z1r_x = {}, z1i_hh = {}, z1i_x = {};
-        f32x16_t z2r_hh = {}, z2r_x = {}, z2i_hh = {}, z2i_x = {};
-        // after every kFoldObs observations the FP32 accumulators move into
-        // FP64 sums in LDS (each lane its own entries, so no barrier)
+        f32x16_t z1r = {}, z1r_x = {}, z1i = {}, z1i_x = {};  // Z1: head x head, cross products
+        f32x16_t z2r = {}, z2i = {};
+        // after every kFoldObs observations the accumulators move into FP32
+        // sums in LDS (each lane its own entries, so no barrier)
         bool folded = false;
         auto fold = [&]() {
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const double p0 = folded ? s_acc[0][v][l] : 0.0, p1 = folded ? s_acc[1][v][l] : 0.0;
-                const double p2 = folded ? s_acc[2][v][l] : 0.0, p3 = folded ? s_acc[3][v][l] : 0.0;
-                s_acc[0][v][l] = p0 + (double)z1r_hh[v] + kTail * (double)z1r_x[v];
-                s_acc[1][v][l] = p1 + (double)z1i_hh[v] + kTail * (double)z1i_x[v];
-                s_acc[2][v][l] = p2 + (double)z2r_hh[v] + kTail * (double)z2r_x[v];
-                s_acc[3][v][l] = p3 + (double)z2i_hh[v] + kTail * (double)z2i_x[v];
+                s_acc[0][v][l] = (folded ? s_acc[0][v][l] : 0.0f) + (z1r[v] + z1r_x[v]);
+                s_acc[1][v][l] = (folded ? s_acc[1][v][l] : 0.0f) + (z1i[v] + z1i_x[v]);
+                s_acc[2][v][l] = (folded ? s_acc[2][v][l] : 0.0f) + z2r[v];
+                s_acc[3][v][l] = (folded ? s_acc[3][v][l] : 0.0f) + z2i[v];
             }
-            z1r_hh = f32x16_t{};
+            z1r = f32x16_t{};
             z1r_x = f32x16_t{};
-            z1i_hh = f32x16_t{};
+            z1i = f32x16_t{};
             z1i_x = f32x16_t{};
-            z2r_hh = f32x16_t{};
-            z2r_x = f32x16_t{};
-            z2i_hh = f32x16_t{};
-            z2i_x = f32x16_t{};
+            z2r = f32x16_t{};
+            z2i = f32x16_t{};
             folded = true;
         };
         for (uint64_t g0 = beg; g0 < end; g0 += kFoldObs) {
@@ -688,46 +690,46 @@ __global__ __launch_bounds__(64) void k_sweep_mfma(SweepArgs a) {
                     qi_l[v] = t;
                 }
                 // Re Z1 = (y u)_r v_r - (y u)_i v_i,  Im Z1 = (y u)_r v_i + (y u)_i v_r
-                LS_MFMA(z1r_hh, ar_h, col[0]);
+                LS_MFMA(z1r, ar_h, col[0]);
                 LS_MFMA(z1r_x, ar_h, col[1]);
                 LS_MFMA(z1r_x, ar_l, col[0]);
-                LS_MFMA(z1r_hh, ai_h, col[4]);
+                LS_MFMA(z1r, ai_h, col[4]);
                 LS_MFMA(z1r_x, ai_h, col[5]);
                 LS_MFMA(z1r_x, ai_l, col[4]);
-                LS_MFMA(z1i_hh, ar_h, col[2]);
+                LS_MFMA(z1i, ar_h, col[2]);
                 LS_MFMA(z1i_x, ar_h, col[3]);
                 LS_MFMA(z1i_x, ar_l, col[2]);
-                LS_MFMA(z1i_hh, ai_h, col[0]);
+                LS_MFMA(z1i, ai_h, col[0]);
                 LS_MFMA(z1i_x, ai_h, col[1]);
                 LS_MFMA(z1i_x, ai_l, col[0]);
                 // the same for Z2 with u^2 and w
-                LS_MFMA(z2r_hh, qr_h, col[6]);
-                LS_MFMA(z2r_x, qr_h, col[7]);
-                LS_MFMA(z2r_x, qr_l, col[6]);
-                LS_MFMA(z2r_hh, qi_h, col[10]);
-                LS_MFMA(z2r_x, qi_h, col[11]);
-                LS_MFMA(z2r_x, qi_l, col[10]);
-                LS_MFMA(z2i_hh, qr_h, col[8]);
-                LS_MFMA(z2i_x, qr_h, col[9]);
-                LS_MFMA(z2i_x, qr_l, col[8]);
-                LS_MFMA(z2i_hh, qi_h, col[6]);
-                LS_MFMA(z2i_x, qi_h, col[7]);
-                LS_MFMA(z2i_x, qi_l, col[6]);
+                LS_MFMA(z2r, qr_h, col[6]);
+                LS_MFMA(z2r, qr_h, col[7]);
+                LS_MFMA(z2r, qr_l, col[6]);
+                LS_MFMA(z2r, qi_h, col[10]);
+                LS_MFMA(z2r, qi_h, col[11]);
+                LS_MFMA(z2r, qi_l, col[10]);
+                LS_MFMA(z2i, qr_h, col[8]);
+                LS_MFMA(z2i, qr_h, col[9]);
+                LS_MFMA(z2i, qr_l, col[8]);
+                LS_MFMA(z2i, qi_h, col[6]);
+                LS_MFMA(z2i, qi_h, col[7]);
+                LS_MFMA(z2i, qi_l, col[6]);
             }
         }
         if (gend < end) fold();
         }
         // the item's FP64 sums of result entry v
         auto sums = [&](int v, double &XC, double &XS, double &C2, double &S2) {
-            XC = (double)z1r_hh[v] + kTail * (double)z1r_x[v];
-            XS = (double)z1i_hh[v] + kTail * (double)z1i_x[v];
-            C2 = (double)z2r_hh[v] + kTail * (double)z2r_x[v];
-            S2 = (double)z2i_hh[v] + kTail * (double)z2i_x[v];
+            XC = (double)z1r[v] + (double)z1r_x[v];
+            XS = (double)z1i[v] + (double)z1i_x[v];
+            C2 = (double)z2r[v];
+            S2 = (double)z2i[v];
             if (folded) {
-                XC += s_acc[0][v][l];
-                XS += s_acc[1][v][l];
-                C2 += s_acc[2][v][l];
-                S2 += s_acc[3][v][l];
+                XC += (double)s_acc[0][v][l];
+                XS += (double)s_acc[1][v][l];
+                C2 += (double)s_acc[2][v][l];
+                S2 += (double)s_acc[3][v][l];
             }
             XC *= kUnscale;
             XS *= kUnscale;
@@ -784,8 +786,8 @@ __global__ __launch_bounds__(64) void k_sweep_mfma(SweepArgs a) {
 //   Zw = sum q^2 e^{i phi},  Zy = sum q (q y_c) e^{i phi},  Z2 = sum q^2 e^{2 i phi}
 // give the seven weighted sums of the generalized power: C, S = Zw;
 // YC, YS = Zy; CC, SS = (Wf +- Re Z2) / 2; CS = Im Z2 / 2, with Wf = sum q^2.
-// 36 MFMAs and 12 accumulators per 16 observations; the FP64 fold holds six
-// sums (48 KiB of LDS per workgroup).
+// 36 MFMAs and 12 accumulators (head x head, cross products) per 16
+// observations; the FP64 fold holds six sums (48 KiB of LDS per workgroup).
 template <typename OUT>
 __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
     constexpr int T = 64, M = 16;
@@ -797,7 +799,6 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
     __shared__ uint32_t s_pi[1][kPeaks];
     __shared__ uint64_t s_item;
     const uint32_t l = threadIdx.x, rc = l & 31u, kh = l >> 5;
-    constexpr double kTail = 1.0 / 2048.0;
     constexpr double kUnscale = 1.0 / ((double)kFacScale * (double)kFacScale);
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
          item = next_item(a, &s_item, item)) {
@@ -830,12 +831,12 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
                 double p[6];
 #pragma unroll
                 for (int q = 0; q < 6; ++q) p[q] = folded ? s_acc[q][v][l] : 0.0;
-                s_acc[0][v][l] = p[0] + (double)zw[0][v] + kTail * (double)zw[1][v];
-                s_acc[1][v][l] = p[1] + (double)zw[2][v] + kTail * (double)zw[3][v];
-                s_acc[2][v][l] = p[2] + (double)zy[0][v] + kTail * (double)zy[1][v];
-                s_acc[3][v][l] = p[3] + (double)zy[2][v] + kTail * (double)zy[3][v];
-                s_acc[4][v][l] = p[4] + (double)z2[0][v] + kTail * (double)z2[1][v];
-                s_acc[5][v][l] = p[5] + (double)z2[2][v] + kTail * (double)z2[3][v];
+                s_acc[0][v][l] = p[0] + (double)zw[0][v] + (double)zw[1][v];
+                s_acc[1][v][l] = p[1] + (double)zw[2][v] + (double)zw[3][v];
+                s_acc[2][v][l] = p[2] + (double)zy[0][v] + (double)zy[1][v];
+                s_acc[3][v][l] = p[3] + (double)zy[2][v] + (double)zy[3][v];
+                s_acc[4][v][l] = p[4] + (double)z2[0][v] + (double)z2[1][v];
+                s_acc[5][v][l] = p[5] + (double)z2[2][v] + (double)z2[3][v];
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -944,12 +945,12 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
         }
         // the item's FP64 sums of result entry v: Re/Im Zw, Zy (y unscaled), Z2
         auto sums = [&](int v, double (&z)[6]) {
-            z[0] = (double)zw[0][v] + kTail * (double)zw[1][v];
-            z[1] = (double)zw[2][v] + kTail * (double)zw[3][v];
-            z[2] = (double)zy[0][v] + kTail * (double)zy[1][v];
-            z[3] = (double)zy[2][v] + kTail * (double)zy[3][v];
-            z[4] = (double)z2[0][v] + kTail * (double)z2[1][v];
-            z[5] = (double)z2[2][v] + kTail * (double)z2[3][v];
+            z[0] = (double)zw[0][v] + (double)zw[1][v];
+            z[1] = (double)zw[2][v] + (double)zw[3][v];
+            z[2] = (double)zy[0][v] + (double)zy[1][v];
+            z[3] = (double)zy[2][v] + (double)zy[3][v];
+            z[4] = (double)z2[0][v] + (double)z2[1][v];
+            z[5] = (double)z2[2][v] + (double)z2[3][v];
             if (folded) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) z[q] += s_acc[q][v][l];

@@ -79,7 +79,8 @@ enum lsamd_form {
     /* as SPLIT but with FP32 arithmetic across frequencies and observations:
      * the default tile (both variants) on the FP16 matrix cores, the sums of
      * 32 x 32 bins as row x column factor products with head/tail splitting,
-     * FP32 accumulation folded into FP64 every 256 observations; the VALU
+     * FP32 accumulation folded every 256 observations (into FP32 sums for
+     * the standard variant, FP64 for the generalized); the VALU
      * tiles FP32 rotation / recurrence and FP32 sums over 32 or 64
      * observations folded into FP64: the fastest form;
      * within 1e-5 of FP64 on well-conditioned data (6.4e-6 on a 1000-object

@@ -311,9 +311,11 @@ def test_observation_slices_single_object(form, variant, dtype):
     p1 = r1["pgram"][0].cpu().numpy()
     pb = rb["pgram"].cpu().numpy()
     assert np.array_equal(pb[0], pb[-1])       # whole-object sweeps are per-object exact
-    # sliced vs whole: FP64 sums regrouped (split) or FP32 partials cut at other
-    # observations (split32); a float periodogram adds one rounding
-    tol = (1e-12 if dtype == F64 else 2e-7) if form == _native.FORM_SPLIT else 2e-6
+    # sliced vs whole: FP64 sums regrouped (split) or FP32 sums cut at other
+    # observations (split32: the standard matrix-core tile folds its partials in
+    # FP32, so each side carries its own FP32 rounding, ~3e-6 apart on this
+    # object); a float periodogram adds one rounding
+    tol = (1e-12 if dtype == F64 else 2e-7) if form == _native.FORM_SPLIT else 5e-6
     assert_close(p1, pb[0], tol, 1e-3, tol)
     assert np.array_equal(r1["index"].cpu().numpy()[0], rb["index"].cpu().numpy()[0])
     if form == _native.FORM_SPLIT or not gen:
