@@ -787,14 +787,16 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
 // give the seven weighted sums of the generalized power: C, S = Zw;
 // YC, YS = Zy; CC, SS = (Wf +- Re Z2) / 2; CS = Im Z2 / 2, with Wf = sum q^2.
 // 36 MFMAs and 12 accumulators (head x head, cross products) per 16
-// observations; the FP64 fold holds six sums (48 KiB of LDS per workgroup).
+// observations.  The fold keeps Zy in FP64 and the weight-only Zw and Z2 in
+// FP32 (34 KiB of LDS: four workgroups per CU, one per SIMD).
 template <typename OUT>
 __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
     constexpr int T = 64, M = 16;
     constexpr uint32_t kFoldObs = 256;
     __shared__ double s_t[T];
     __shared__ float s_q[T], s_y[T];
-    __shared__ double s_acc[6][16][T];  // FP64 Re/Im of Zw, Zy, Z2 of a lane's 16 bins
+    __shared__ double s_accy[2][16][T];  // FP64 Re/Im of Zy of a lane's 16 bins
+    __shared__ float s_accw[4][16][T];   // FP32 Re/Im of Zw and Z2 (weights only)
     __shared__ double s_pp[1][kPeaks];
     __shared__ uint32_t s_pi[1][kPeaks];
     __shared__ uint64_t s_item;
@@ -828,15 +830,16 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
         auto fold = [&]() {
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                double p[6];
+                float pw[4];
 #pragma unroll
-                for (int q = 0; q < 6; ++q) p[q] = folded ? s_acc[q][v][l] : 0.0;
-                s_acc[0][v][l] = p[0] + (double)zw[0][v] + (double)zw[1][v];
-                s_acc[1][v][l] = p[1] + (double)zw[2][v] + (double)zw[3][v];
-                s_acc[2][v][l] = p[2] + (double)zy[0][v] + (double)zy[1][v];
-                s_acc[3][v][l] = p[3] + (double)zy[2][v] + (double)zy[3][v];
-                s_acc[4][v][l] = p[4] + (double)z2[0][v] + (double)z2[1][v];
-                s_acc[5][v][l] = p[5] + (double)z2[2][v] + (double)z2[3][v];
+                for (int q = 0; q < 4; ++q) pw[q] = folded ? s_accw[q][v][l] : 0.0f;
+                const double py0 = folded ? s_accy[0][v][l] : 0.0, py1 = folded ? s_accy[1][v][l] : 0.0;
+                s_accw[0][v][l] = pw[0] + (zw[0][v] + zw[1][v]);
+                s_accw[1][v][l] = pw[1] + (zw[2][v] + zw[3][v]);
+                s_accy[0][v][l] = py0 + (double)zy[0][v] + (double)zy[1][v];
+                s_accy[1][v][l] = py1 + (double)zy[2][v] + (double)zy[3][v];
+                s_accw[2][v][l] = pw[2] + (z2[0][v] + z2[1][v]);
+                s_accw[3][v][l] = pw[3] + (z2[2][v] + z2[3][v]);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -953,7 +956,11 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
             z[5] = (double)z2[2][v] + (double)z2[3][v];
             if (folded) {
 #pragma unroll
-                for (int q = 0; q < 6; ++q) z[q] += s_acc[q][v][l];
+                for (int q = 0; q < 2; ++q) {
+                    z[q] += s_accw[q][v][l];
+                    z[2 + q] += s_accy[q][v][l];
+                    z[4 + q] += s_accw[2 + q][v][l];
+                }
             }
             const double iys = 1.0 / (double)ys;
 #pragma unroll
