@@ -61,6 +61,7 @@ class SweepArgs(Structure):
         ("offsets", c_void_p), ("order", c_void_p), ("t", c_void_p), ("y", c_void_p),
         ("dy", c_void_p), ("pgram", c_void_p), ("peak_index", c_void_p), ("peak_power", c_void_p),
         ("peak_period", c_void_p), ("schedule", c_int32), ("obs_slices", c_int32),
+        ("tile_threads", c_int32), ("tile_fpt", c_int32),
     ]
 
 

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -82,7 +83,8 @@ template void precenter_host<float>(float *, const float *, uint64_t);
 template void precenter_host<double>(double *, const double *, uint64_t);
 
 // ---------------------------------------------------------------------------
-// device context: one stream and grow-only buffers, reused across calls
+// device contexts: per device, one stream and grow-only buffers, reused
+// across calls
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -104,9 +106,31 @@ struct Ctx {
     hipEvent_t slice[2] = {nullptr, nullptr};  // sweep of a slice done (sweep stream)
 };
 
-Ctx &ctx() {
-    static Ctx *c = new Ctx();  // never destroyed: HIP may be torn down first at exit
-    return *c;
+// One context per device, for the device current on the calling thread.
+// Calls on one device serialise on its context; calls on different devices
+// (one host thread each) run side by side.  Contexts are never destroyed:
+// HIP may be torn down first at exit.
+Ctx *ctx_for_current_device(int *err) {
+    static std::mutex mu;
+    static std::map<int, Ctx *> *table = new std::map<int, Ctx *>();
+    int dev = 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        *err = set_error(LSAMD_ERR_NODEVICE, "no HIP device visible");
+        return nullptr;
+    }
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        *err = set_error(LSAMD_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    Ctx *&c = (*table)[dev];
+    if (!c) {
+        c = new Ctx();
+        c->device = dev;
+    }
+    return c;
 }
 
 #define HIP_OK(call)                                                                          \
@@ -119,10 +143,6 @@ Ctx &ctx() {
 
 int ctx_init(Ctx &c) {
     if (c.ready) return LSAMD_OK;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return set_error(LSAMD_ERR_NODEVICE, "no HIP device visible");
-    HIP_OK(hipGetDevice(&c.device));
     HIP_OK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&c.copy_stream, hipStreamNonBlocking));
     for (int i = 0; i < 2; ++i) {
@@ -271,9 +291,12 @@ int lsamd_batch_host(int variant, int dtype, int form, int normalize, int precen
         }
     }
 
-    Ctx &c = ctx();
+    int rc = LSAMD_OK;
+    Ctx *cp = ctx_for_current_device(&rc);
+    if (!cp) return rc;
+    Ctx &c = *cp;
     std::lock_guard<std::mutex> lock(c.mu);
-    int rc = ctx_init(c);
+    rc = ctx_init(c);
     if (rc) return rc;
 
     // memory budget for the periodogram (computeNumBatches, main.cu:423-466):

@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -321,10 +322,11 @@ const TileEntry *find_tile(int form, int variant, int dtype, int T, int M) {
     return nullptr;
 }
 
-// user overrides: [form][variant] -> (T, M); 0 = built-in
+// Process-wide default tile per [form][variant], packed as T << 16 | M
+// (0 = built-in), for sweeps whose args leave the tile at 0.  Sweeps read it
+// once when they are planned; per-call tiles never touch it.
 constexpr int kForms = 3;
-int g_tile_T[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
-int g_tile_M[kForms][2] = {{0, 0}, {0, 0}, {0, 0}};
+std::atomic<uint32_t> g_default_tile[kForms][2];
 
 // Built-in shapes: the fastest of the tile sweep on the 1000-object x 1e6
 // batches (MI355X): split 256x16 (standard) / 256x8 (generalized); split32
@@ -336,10 +338,42 @@ Tile default_tile(int form, int variant) {
     return Tile{256, 1};
 }
 
-bool g_timing = false;
-double g_last_ms = -1.0;
-hipEvent_t g_ev[2] = {nullptr, nullptr};
-bool g_ev_pending = false;
+// Kernel timing, per host thread: the events bracketing this thread's last
+// timed sweep (created on the device of that sweep's stream).
+struct Timing {
+    bool enabled = false;
+    double last_ms = -1.0;
+    int device = -1;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending = false;
+};
+thread_local Timing t_timing;
+
+// Events of the calling thread on the current device, created on first use.
+hipError_t timing_events(Timing &tm) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (tm.ev[0] && tm.device == dev) return hipSuccess;
+    for (hipEvent_t &ev : tm.ev) {
+        if (ev) (void)hipEventDestroy(ev);
+        ev = nullptr;
+    }
+    tm.pending = false;
+    for (hipEvent_t &ev : tm.ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
+    tm.device = dev;
+    return hipSuccess;
+}
+
+// Events owned by one scope (the autotuner's trial timings).
+struct ScopedEvents {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~ScopedEvents() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -348,8 +382,14 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 Tile current_tile(int form, int variant, int dtype) {
     (void)dtype;
     if (form < 0 || form >= kForms || variant < 0 || variant > 1) return Tile{0, 0};
-    if (g_tile_T[form][variant] > 0) return Tile{g_tile_T[form][variant], g_tile_M[form][variant]};
+    const uint32_t v = g_default_tile[form][variant].load(std::memory_order_relaxed);
+    if (v) return Tile{(int)(v >> 16), (int)(v & 0xFFFFu)};
     return default_tile(form, variant);
+}
+
+Tile tile_of(const lsamd_sweep_args *a) {
+    if (a->tile_threads > 0 && a->tile_fpt > 0) return Tile{a->tile_threads, a->tile_fpt};
+    return current_tile(a->form, a->variant, a->dtype);
 }
 
 bool tile_supported(int form, int variant, int threads, int fpt) {
@@ -359,7 +399,7 @@ bool tile_supported(int form, int variant, int threads, int fpt) {
 
 int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
     *w = Workspace{};
-    const Tile tile = current_tile(a->form, a->variant, a->dtype);
+    const Tile tile = tile_of(a);
     w->tile = tile;
     const uint64_t per_chunk = (uint64_t)tile.threads * tile.fpt;
     w->chunks = (uint32_t)((a->num_freqs + per_chunk - 1) / per_chunk);
@@ -431,14 +471,13 @@ int lsamd_set_tile(int form, int variant, int threads, int fpt) {
     if (form < 0 || form >= kForms || variant < 0 || variant > 1)
         return set_error(LSAMD_ERR_INVALID, "lsamd_set_tile: bad form/variant");
     if (threads == 0 || fpt == 0) {
-        g_tile_T[form][variant] = g_tile_M[form][variant] = 0;
+        g_default_tile[form][variant].store(0u);
         return LSAMD_OK;
     }
     if (!tile_supported(form, variant, threads, fpt))
         return set_error(LSAMD_ERR_INVALID, "lsamd_set_tile: tile shape " + std::to_string(threads) +
                                                 "x" + std::to_string(fpt) + " not compiled");
-    g_tile_T[form][variant] = threads;
-    g_tile_M[form][variant] = fpt;
+    g_default_tile[form][variant].store((uint32_t)threads << 16 | (uint32_t)fpt);
     return LSAMD_OK;
 }
 
@@ -448,15 +487,16 @@ void lsamd_get_tile(int form, int variant, int dtype, int *threads, int *fpt) {
     if (fpt) *fpt = t.fpt;
 }
 
-void lsamd_set_timing(int enabled) { g_timing = enabled != 0; }
+void lsamd_set_timing(int enabled) { t_timing.enabled = enabled != 0; }
 
 double lsamd_last_sweep_kernel_ms(void) {
-    if (g_ev_pending) {
+    Timing &tm = t_timing;
+    if (tm.pending) {
         float ms = -1.0f;
-        if (hipEventElapsedTime(&ms, g_ev[0], g_ev[1]) == hipSuccess) g_last_ms = ms;
-        g_ev_pending = false;
+        if (hipEventElapsedTime(&ms, tm.ev[0], tm.ev[1]) == hipSuccess) tm.last_ms = ms;
+        tm.pending = false;
     }
-    return g_last_ms;
+    return tm.last_ms;
 }
 
 #define LS_HIP(call)                                                                         \
@@ -491,6 +531,10 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: bad schedule");
     if (a->obs_slices != 0 && a->obs_slices != 1)
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: obs_slices must be 0 or 1");
+    if ((a->tile_threads != 0 || a->tile_fpt != 0) &&
+        !find_tile(a->form, a->variant, a->dtype, a->tile_threads, a->tile_fpt))
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: tile shape " + std::to_string(a->tile_threads) +
+                                                "x" + std::to_string(a->tile_fpt) + " not compiled");
     Workspace w;
     plan_workspace(a, &w);
     if (!workspace || workspace_bytes < w.bytes)
@@ -619,13 +663,11 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     }
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     LS_HIP(hipStreamIsCapturing(st, &cap));
-    const bool timing = g_timing && cap == hipStreamCaptureStatusNone;
+    Timing &tm = t_timing;
+    const bool timing = tm.enabled && cap == hipStreamCaptureStatusNone;
     if (timing) {
-        if (!g_ev[0]) {
-            LS_HIP(hipEventCreate(&g_ev[0]));
-            LS_HIP(hipEventCreate(&g_ev[1]));
-        }
-        LS_HIP(hipEventRecord(g_ev[0], st));
+        LS_HIP(timing_events(tm));
+        LS_HIP(hipEventRecord(tm.ev[0], st));
     }
     if (te->prep) {  // per-object set-up of the tile, inside the timed sweep
         // wavefronts per object: enough for the mean object's 16-observation
@@ -659,8 +701,8 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
                                dim3((uint32_t)((uint64_t)K * w.chunks)), dim3(T), kargs, 0, st));
     }
     if (timing) {
-        LS_HIP(hipEventRecord(g_ev[1], st));
-        g_ev_pending = true;
+        LS_HIP(hipEventRecord(tm.ev[1], st));
+        tm.pending = true;
     }
     (void)M;
 
@@ -730,17 +772,15 @@ int lsamd_graph_create(const lsamd_sweep_args *a, void *workspace, size_t worksp
 int lsamd_graph_launch(lsamd_graph *g, void *stream_v) {
     if (!g || !g->exec) return set_error(LSAMD_ERR_INVALID, "lsamd_graph_launch: null graph");
     hipStream_t st = static_cast<hipStream_t>(stream_v);
-    if (g_timing) {
-        if (!g_ev[0]) {
-            LS_HIP(hipEventCreate(&g_ev[0]));
-            LS_HIP(hipEventCreate(&g_ev[1]));
-        }
-        LS_HIP(hipEventRecord(g_ev[0], st));
+    Timing &tm = t_timing;
+    if (tm.enabled) {
+        LS_HIP(timing_events(tm));
+        LS_HIP(hipEventRecord(tm.ev[0], st));
     }
     LS_HIP(hipGraphLaunch(g->exec, st));
-    if (g_timing) {
-        LS_HIP(hipEventRecord(g_ev[1], st));
-        g_ev_pending = true;
+    if (tm.enabled) {
+        LS_HIP(hipEventRecord(tm.ev[1], st));
+        tm.pending = true;
     }
     return LSAMD_OK;
 }
@@ -767,22 +807,19 @@ static lsamd_sweep_args autotune_sample(const lsamd_sweep_args *a) {
 
 size_t lsamd_autotune_workspace_size(const lsamd_sweep_args *a) {
     if (!a || a->form < 0 || a->form >= kForms || a->variant < 0 || a->variant > 1) return 0;
-    const lsamd_sweep_args s = autotune_sample(a);
-    const int saved_T = g_tile_T[a->form][a->variant], saved_M = g_tile_M[a->form][a->variant];
+    lsamd_sweep_args s = autotune_sample(a);
     size_t need = 0;
     int ntab = 0;
     const TileEntry *tab = tiles_of(a->form, &ntab);
     for (int i = 0; i < ntab; ++i) {
         const TileEntry &te = tab[i];
         if (te.form != a->form || te.variant != a->variant || te.dtype != a->dtype) continue;
-        g_tile_T[a->form][a->variant] = te.T;
-        g_tile_M[a->form][a->variant] = te.M;
+        s.tile_threads = te.T;
+        s.tile_fpt = te.M;
         Workspace w;
         plan_workspace(&s, &w);
         need = std::max(need, w.bytes);
     }
-    g_tile_T[a->form][a->variant] = saved_T;
-    g_tile_M[a->form][a->variant] = saved_M;
     return need;
 }
 
@@ -793,52 +830,47 @@ int lsamd_autotune(const lsamd_sweep_args *a, void *workspace, size_t workspace_
         return set_error(LSAMD_ERR_INVALID, "lsamd_autotune: bad form/variant");
     hipStream_t st = static_cast<hipStream_t>(stream_v);
     lsamd_sweep_args s = autotune_sample(a);
-    const int form = a->form, variant = a->variant;
-    const int saved_T = g_tile_T[form][variant], saved_M = g_tile_M[form][variant];
-    const bool saved_timing = g_timing;
-    hipEvent_t e0, e1;
-    LS_HIP(hipEventCreate(&e0));
-    LS_HIP(hipEventCreate(&e1));
+    // the trial sweeps are timed here, not by the caller's thread timing
+    Timing &tm = t_timing;
+    const bool saved_timing = tm.enabled;
+    tm.enabled = false;
+    struct Restore {
+        Timing &tm;
+        bool on;
+        ~Restore() { tm.enabled = on; }
+    } restore{tm, saved_timing};
+    ScopedEvents ev;
+    LS_HIP(hipEventCreate(&ev.ev[0]));
+    LS_HIP(hipEventCreate(&ev.ev[1]));
     float best_ms = 1e30f;
     int best_T = 0, best_M = 0;
     int ntab = 0;
-    const TileEntry *tab = tiles_of(form, &ntab);
+    const TileEntry *tab = tiles_of(a->form, &ntab);
     for (int i = 0; i < ntab; ++i) {
         const TileEntry &te = tab[i];
-        if (te.form != form || te.variant != variant || te.dtype != a->dtype) continue;
-        g_tile_T[form][variant] = te.T;
-        g_tile_M[form][variant] = te.M;
+        if (te.form != a->form || te.variant != a->variant || te.dtype != a->dtype) continue;
+        s.tile_threads = te.T;
+        s.tile_fpt = te.M;
         Workspace w;
         plan_workspace(&s, &w);
         if (w.bytes > workspace_bytes) continue;
-        g_timing = false;
         int rc = lsamd_sweep(&s, workspace, workspace_bytes, stream_v);  // warm-up
-        if (rc) break;
-        LS_HIP(hipEventRecord(e0, st));
+        if (rc) return rc;
+        LS_HIP(hipEventRecord(ev.ev[0], st));
         rc = lsamd_sweep(&s, workspace, workspace_bytes, stream_v);
-        if (rc) break;
-        LS_HIP(hipEventRecord(e1, st));
-        LS_HIP(hipEventSynchronize(e1));
+        if (rc) return rc;
+        LS_HIP(hipEventRecord(ev.ev[1], st));
+        LS_HIP(hipEventSynchronize(ev.ev[1]));
         float ms = 0.0f;
-        LS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        LS_HIP(hipEventElapsedTime(&ms, ev.ev[0], ev.ev[1]));
         if (ms < best_ms) {
             best_ms = ms;
             best_T = te.T;
             best_M = te.M;
         }
     }
-    g_timing = saved_timing;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (best_T == 0) {
-        g_tile_T[form][variant] = saved_T;
-        g_tile_M[form][variant] = saved_M;
-        return lsamd_last_error_code() ? lsamd_last_error_code()
-                                       : set_error(LSAMD_ERR_INVALID, "lsamd_autotune: no tile fits "
-                                                                      "the workspace");
-    }
-    g_tile_T[form][variant] = best_T;
-    g_tile_M[form][variant] = best_M;
+    if (best_T == 0)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_autotune: no tile fits the workspace");
     if (threads_out) *threads_out = best_T;
     if (fpt_out) *fpt_out = best_M;
     return LSAMD_OK;

@@ -43,12 +43,15 @@ class SweepPlan:
     int32/uint32 CUDA tensor of K object numbers) is the processing order;
     by default objects are processed longest first.  obs_slices=1 keeps a
     single-object split-form sweep from spreading its observations over
-    workgroups (0, the default, lets it).
+    workgroups (0, the default, lets it).  tile=(threads, freqs_per_thread)
+    picks a compiled tile shape for this plan only (None: the process
+    default); autotune() sets it to the fastest.
     """
 
     def __init__(self, t, y, dy, offsets, *, min_freq, max_freq, num_freqs, variant,
                  form=_native.FORM_SPLIT, normalize=True, precenter=True, pgram=True, top_k=1,
-                 order="longest_first", schedule=_native.SCHEDULE_GRID, obs_slices=0, lib=None):
+                 order="longest_first", schedule=_native.SCHEDULE_GRID, obs_slices=0, tile=None,
+                 lib=None):
         if not t.is_cuda:
             raise RuntimeError("SweepPlan needs CUDA (HIP) tensors")
         self.lib = lib or _native.core()
@@ -87,6 +90,8 @@ class SweepPlan:
             a.peak_period = self.peak_period.data_ptr()
         a.schedule = int(schedule)
         a.obs_slices = int(obs_slices)
+        if tile is not None:
+            a.tile_threads, a.tile_fpt = int(tile[0]), int(tile[1])
         self.args = a
         ws = int(self.lib.lsamd_sweep_workspace_size(byref(a)))
         self.workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
@@ -96,9 +101,20 @@ class SweepPlan:
     def _stream(self, stream):
         return stream if stream is not None else torch.cuda.current_stream(self.t.device)
 
+    def tile(self):
+        """(threads, freqs_per_thread) of this plan's sweep kernel."""
+        from ctypes import c_int
+        a = self.args
+        if a.tile_threads and a.tile_fpt:
+            return a.tile_threads, a.tile_fpt
+        T, M = c_int(), c_int()
+        self.lib.lsamd_get_tile(a.form, a.variant, a.dtype, byref(T), byref(M))
+        return T.value, M.value
+
     def autotune(self, stream=None):
         """Pick the fastest compiled tile shape for this plan's form/variant
-        (lsamd_autotune) and resize the workspace for it; returns (T, M)."""
+        (lsamd_autotune), keep it for this plan and resize the workspace for
+        it; returns (T, M)."""
         from ctypes import c_int
         need = int(self.lib.lsamd_autotune_workspace_size(byref(self.args)))
         big = torch.empty(max(need, 256), dtype=torch.uint8, device=self.t.device)
@@ -107,6 +123,7 @@ class SweepPlan:
                                      c_void_p(self._stream(stream).cuda_stream), byref(T), byref(M))
         _native.check(self.lib, rc)
         del big
+        self.args.tile_threads, self.args.tile_fpt = T.value, M.value
         ws = int(self.lib.lsamd_sweep_workspace_size(byref(self.args)))
         self.workspace = torch.empty(ws, dtype=torch.uint8, device=self.t.device)
         self.destroy_graph()

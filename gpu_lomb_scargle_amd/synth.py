@@ -10,7 +10,9 @@ Workloads (angular frequency grid 1.005 .. 150.796 rad/day, max excluded):
   * lognormal_batch         K objects, N ~ lognormal(median, sigma) clipped
     - standard, 1000 x median 140 / sigma 0.6 / clip 20..1000, 1e6 bins
     - generalized ("with_error"): span 2000 d, y = m + A sin(...) + N(0, dmag),
-      m ~ U(15, 20) not subtracted, dmag ~ logU(0.01, 0.2) per object, A = 0.3
+      m ~ U(15, 20) not subtracted, dmag ~ logU(0.01, 0.2) per object, A = 0.3;
+      with dmag_per_obs each observation draws its own dmag ~ logU(0.01, 0.2)
+      (weights 1/dmag^2 spread over a factor of 400 within an object)
     - uneven, 100000 x median 60 / sigma 1.0 / clip 5..5000, 2e5 bins
     - refinement, 4096 x median 200 / sigma 0.5 / clip >= 5, 5e5 bins
 """
@@ -23,16 +25,18 @@ OMEGA_MAX = 150.796
 AMPLITUDE_WITH_ERROR = 0.3
 
 
-def _one(rng, n, span, with_error):
+def _one(rng, n, span, with_error, dmag_per_obs=False):
     nights = rng.integers(0, span + 1, n)
     t = np.sort(nights + rng.uniform(0.0, 0.3, n))
     P = float(np.exp(rng.uniform(np.log(0.05), np.log(10.0))))
     phi = float(rng.uniform(0.0, 2.0 * np.pi))
     if with_error:
         m = float(rng.uniform(15.0, 20.0))
-        dmag = float(np.exp(rng.uniform(np.log(0.01), np.log(0.2))))
-        y = m + AMPLITUDE_WITH_ERROR * np.sin(2.0 * np.pi * t / P + phi) + rng.normal(0.0, dmag, n)
-        dy = np.full(n, dmag)
+        if dmag_per_obs:
+            dy = np.exp(rng.uniform(np.log(0.01), np.log(0.2), n))
+        else:
+            dy = np.full(n, float(np.exp(rng.uniform(np.log(0.01), np.log(0.2)))))
+        y = m + AMPLITUDE_WITH_ERROR * np.sin(2.0 * np.pi * t / P + phi) + rng.normal(0.0, dy, n)
     else:
         y = np.sin(2.0 * np.pi * t / P + phi) + rng.normal(0.0, 0.3, n)
         y = (y - y.mean()) / y.std()
@@ -56,13 +60,13 @@ def lognormal_counts(rng, K, median, sigma, lo=None, hi=None):
 
 
 def lognormal_batch(K=1000, median=140, sigma=0.6, lo=20, hi=1000, span=1800, with_error=False,
-                    seed=1001):
+                    seed=1001, dmag_per_obs=False):
     """K objects with log-normal observation counts, rows object-contiguous."""
     rng = np.random.default_rng(seed)
     counts = lognormal_counts(rng, K, median, sigma, lo, hi)
     ts, ys, dys, Ps = [], [], [], []
     for k in range(K):
-        t, y, dy, P = _one(rng, int(counts[k]), span, with_error)
+        t, y, dy, P = _one(rng, int(counts[k]), span, with_error, dmag_per_obs)
         ts.append(t)
         ys.append(y)
         dys.append(dy)
@@ -83,9 +87,10 @@ def standard_batch_1000(seed=1001):
     return lognormal_batch(1000, 140, 0.6, 20, 1000, 1800, False, seed)
 
 
-def generalized_batch_1000(seed=1002):
-    """The 1000-object layout over 2000 d, un-centred magnitudes with dmag."""
-    return lognormal_batch(1000, 140, 0.6, 20, 1000, 2000, True, seed)
+def generalized_batch_1000(seed=1002, dmag_per_obs=False):
+    """The 1000-object layout over 2000 d, un-centred magnitudes with dmag
+    (per object, or per observation with dmag_per_obs)."""
+    return lognormal_batch(1000, 140, 0.6, 20, 1000, 2000, True, seed, dmag_per_obs)
 
 
 def uneven_batch_100000(seed=1003, with_error=False):

@@ -78,11 +78,11 @@ enum lsamd_form {
     LSAMD_FORM_SPLIT = 1,
     /* as SPLIT but with FP32 arithmetic across frequencies and observations:
      * the default tile (both variants) on the FP16 matrix cores, the sums of
-     * 32 x 32 bins as row x column factor products with head/tail splitting,
-     * FP32 accumulation folded every 256 observations (into FP32 sums for
-     * the standard variant, FP64 for the generalized); the VALU
-     * tiles FP32 rotation / recurrence and FP32 sums over 32 or 64
-     * observations folded into FP64: the fastest form;
+     * 32 x 32 bins as row x column factor products with head/tail splitting
+     * and FP32 accumulation, folded every 256 observations (standard: into
+     * FP32 sums; generalized: the y sums into FP64, the weight-only sums
+     * into FP32); the VALU tiles FP32 rotation / recurrence and FP32 sums
+     * over 32 or 64 observations folded into FP64: the fastest form;
      * within 1e-5 of FP64 on well-conditioned data (6.4e-6 on a 1000-object
      * standard batch), up to ~3e-5 on generalized data and ~1e-4 at strong
      * aliases */
@@ -135,6 +135,12 @@ typedef struct lsamd_sweep_args {
     int32_t obs_slices;      /* split forms, 0: a sweep of one object with many observations
                               * spreads them over workgroups (results then differ from the
                               * same object swept in a batch by rounding only); 1: never */
+    /* Tile shape of this sweep (threads per workgroup, frequencies per
+     * thread), one of the shapes compiled for form/variant (lsamd_autotune
+     * reports the fastest); 0, 0: the process default (lsamd_set_tile, else
+     * the built-in choice). */
+    int32_t tile_threads;
+    int32_t tile_fpt;
 } lsamd_sweep_args;
 
 /* Bytes of device workspace lsamd_sweep needs for these arguments. */
@@ -160,9 +166,10 @@ int lsamd_graph_launch(lsamd_graph *graph, void *stream);
 void lsamd_graph_destroy(lsamd_graph *graph);
 
 /* Time every compiled tile shape of args->form/variant/dtype on these
- * arguments (reduced to at most 65536 frequency bins, peaks only), keep the
- * fastest for later sweeps (as lsamd_set_tile would) and report it.  Uses
- * the caller's workspace; synchronises the stream. */
+ * arguments (reduced to at most 65536 frequency bins, peaks only) and report
+ * the fastest; the caller puts it into args->tile_threads / tile_fpt.  No
+ * process-wide setting changes.  Uses the caller's workspace; synchronises
+ * the stream. */
 int lsamd_autotune(const lsamd_sweep_args *args, void *workspace, size_t workspace_bytes,
                    void *stream, int *threads, int *freqs_per_thread);
 /* Workspace bytes lsamd_autotune needs for these arguments. */
@@ -277,11 +284,11 @@ void lsamd_clear_error(void);
 /* Number of visible HIP devices (0 when none; never fails). */
 int lsamd_device_count(void);
 
-/* When enabled, lsamd_sweep brackets its dominant kernel with HIP events on
- * its stream; lsamd_last_sweep_kernel_ms() returns that duration after the
- * stream has been synchronised (negative if unavailable).  Process-wide
- * state, meant for benchmarks: with several threads sweeping at once the
- * reported duration is that of whichever sweep recorded last. */
+/* When enabled for the calling thread, lsamd_sweep and lsamd_graph_launch
+ * bracket the sweep kernels with HIP events on their stream;
+ * lsamd_last_sweep_kernel_ms() returns the duration of this thread's last
+ * bracketed sweep once its stream has been synchronised (negative if
+ * unavailable).  The setting and the events are per host thread. */
 void lsamd_set_timing(int enabled);
 double lsamd_last_sweep_kernel_ms(void);
 
@@ -290,11 +297,12 @@ double lsamd_last_sweep_kernel_ms(void);
  * threads, 1..16 frequencies).  SPLIT32 64 x 16 (the built-in choice) is the
  * matrix-core kernel; 256 x 8 (standard) and 256 x 4 (generalized) the
  * fastest VALU ones.  Zero restores the built-in choice.  Returns
- * LSAMD_ERR_INVALID for shapes not compiled.  Process-wide setting (as is
- * the choice lsamd_autotune makes); set it before sweeping from several
- * threads, not while they run. */
+ * LSAMD_ERR_INVALID for shapes not compiled.  This is the process-wide
+ * default for sweeps whose args leave tile_threads / tile_fpt at 0; a sweep
+ * reads it once, when it is planned.  Per-call tiles (the args fields) are
+ * the way to run different shapes from several threads. */
 int lsamd_set_tile(int form, int variant, int threads, int freqs_per_thread);
-/* Tile shape lsamd_sweep would use now (for reports). */
+/* Tile shape lsamd_sweep uses for args that leave the tile at 0 (for reports). */
 void lsamd_get_tile(int form, int variant, int dtype, int *threads, int *freqs_per_thread);
 
 /* Library version (major*10000 + minor*100 + patch). */

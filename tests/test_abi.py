@@ -115,6 +115,34 @@ def test_observation_slices_workspace():
     assert b"obs_slices" in L.lsamd_last_error()
 
 
+def test_per_call_tile_overrides_the_default():
+    """A tile in the args plans the workspace exactly like the same process
+    default set by lsamd_set_tile, leaves that default alone, and an
+    uncompiled per-call tile is rejected before anything is launched."""
+    from ctypes import c_int
+    L = _native.core()
+    a = _native.SweepArgs()
+    a.variant, a.dtype, a.form = _native.STANDARD, _native.F64, _native.FORM_SPLIT32
+    a.num_objects, a.num_freqs, a.num_obs, a.top_k, a.obs_slices = 1000, 1000000, 163652, 1, 1
+    a.min_freq, a.max_freq = 1.005, 150.796
+    try:
+        assert L.lsamd_set_tile(a.form, a.variant, 256, 8) == _native.OK
+        by_default = L.lsamd_sweep_workspace_size(byref(a))
+    finally:
+        L.lsamd_set_tile(a.form, a.variant, 0, 0)
+    builtin = L.lsamd_sweep_workspace_size(byref(a))
+    a.tile_threads, a.tile_fpt = 256, 8
+    assert L.lsamd_sweep_workspace_size(byref(a)) == by_default != builtin
+    T, M = c_int(), c_int()
+    L.lsamd_get_tile(a.form, a.variant, a.dtype, byref(T), byref(M))
+    assert (T.value, M.value) == (64, 16)
+    a.tile_threads, a.tile_fpt = 96, 3
+    a.offsets = a.t = a.y = 1  # never dereferenced: rejected before any launch
+    a.peak_index = 1
+    assert L.lsamd_sweep(byref(a), c_void_p(4096), 1 << 40, None) == _native.ERR_INVALID
+    assert b"not compiled" in L.lsamd_last_error()
+
+
 def test_matrix_core_tile_workspace():
     """The split32 standard matrix-core tile (64 x 16) reserves its column-
     factor fragments: 12 fragments x 64 lanes x 16 B per 16-observation block,

@@ -195,8 +195,19 @@ def test_golden_fixtures_on_gpu(golden, name, dtype):
                                  (got_fix, golden[key + "_fixed_values"])):
                 rel = np.abs(got_v - ref_v) / np.abs(ref_v)
                 assert np.median(rel) < 1e-6 and np.nanmax(rel) < 2e-2
-        same = np.mean(r["index"][:, 0] == golden[key + "_argmax"])
-        assert same >= (1.0 if name == "double" else 0.97)
+        # the same best bin as the fixture wherever the reference output's
+        # two best bins differ by more than the form's tolerance (the full
+        # reference periodogram comes from the oracle, which is bit-equal to
+        # the reference build and has the fixture's argmax)
+        ref_pg = oracle.batch(o, t, y, dy if err else None, TWO_PI * 0.1, TWO_PI * 10.0, 20000, err, dtype,
+                              nthreads=8)[0].astype(F64)
+        assert np.array_equal(np.argmax(ref_pg, axis=1), golden[key + "_argmax"])
+        tol = 10 * TOL[name][0] if name == "double" else 2e-2 if err else 1e-4
+        top2 = np.sort(ref_pg, axis=1)[:, -2:]
+        sep = top2[:, 1] - top2[:, 0] > tol * top2[:, 1]
+        agree = r["index"][:, 0] == golden[key + "_argmax"]
+        print(f"{key}: best bin equal on {int(agree.sum())}/{agree.size}, separated {int(sep.sum())}")
+        assert agree[sep].all()
 
 
 # --------------------------------------------------------------------------
@@ -700,28 +711,76 @@ def test_graph_replays_match_eager(schedule):
 
 
 def test_autotune_picks_a_compiled_tile():
+    """The autotuner reports a compiled tile that the plan then uses; the
+    process-wide default tile stays as it was."""
     torch = pytest.importorskip("torch")
     from ctypes import byref, c_int
     from gpu_lomb_scargle_amd import device
     L = _native.core()
     b = synth.lognormal_batch(200, 140, 0.6, 20, 1000, 1800, False, 46)
     t, y, dy, off = device.numpy_to_device(b["obj_id"], b["t"], b["y"], None, F64)
-    try:
-        plan = device.SweepPlan(t, y, None, off, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
-                                num_freqs=40000, variant=_native.STANDARD, form=_native.FORM_SPLIT32,
-                                top_k=1)
-        T, M = plan.autotune()
-        Tg, Mg = c_int(), c_int()
-        L.lsamd_get_tile(_native.FORM_SPLIT32, _native.STANDARD, _native.F64, byref(Tg), byref(Mg))
-        assert (T, M) == (Tg.value, Mg.value) and T in (64, 128, 256, 512) and 1 <= M <= 16
-        plan.run()
-        torch.cuda.synchronize()
-        ref = oracle.batch(b["obj_id"][:b["counts"][:20].sum()], b["t"][:b["counts"][:20].sum()],
-                           b["y"][:b["counts"][:20].sum()], None, synth.OMEGA_MIN, synth.OMEGA_MAX,
-                           40000, False, F64, nthreads=8)
-        assert_close(plan.results()["pgram"][:20].cpu().numpy(), ref[0], 1e-5, 1e-3, 2e-6)
-    finally:
-        L.lsamd_set_tile(_native.FORM_SPLIT32, _native.STANDARD, 0, 0)
+    plan = device.SweepPlan(t, y, None, off, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
+                            num_freqs=40000, variant=_native.STANDARD, form=_native.FORM_SPLIT32,
+                            top_k=1)
+    T, M = plan.autotune()
+    assert plan.tile() == (T, M) and T in (64, 128, 256, 512) and 1 <= M <= 16
+    Tg, Mg = c_int(), c_int()
+    L.lsamd_get_tile(_native.FORM_SPLIT32, _native.STANDARD, _native.F64, byref(Tg), byref(Mg))
+    assert (Tg.value, Mg.value) == (64, 16)  # the built-in default is untouched
+    plan.run()
+    torch.cuda.synchronize()
+    ref = oracle.batch(b["obj_id"][:b["counts"][:20].sum()], b["t"][:b["counts"][:20].sum()],
+                       b["y"][:b["counts"][:20].sum()], None, synth.OMEGA_MIN, synth.OMEGA_MAX,
+                       40000, False, F64, nthreads=8)
+    assert_close(plan.results()["pgram"][:20].cpu().numpy(), ref[0], 1e-5, 1e-3, 2e-6)
+
+
+def test_concurrent_threads_with_different_tiles():
+    """Two host threads sweep at the same time on their own streams with
+    different per-call tiles (matrix-core and VALU split32) and with kernel
+    timing on; each thread's results equal the same plan run alone, and each
+    thread reads its own kernel time."""
+    import threading
+
+    torch = pytest.importorskip("torch")
+    from gpu_lomb_scargle_amd import device
+    L = _native.core()
+    b = synth.lognormal_batch(64, 140, 0.6, 20, 1000, 1800, False, 47)
+    t, y, dy, off = device.numpy_to_device(b["obj_id"], b["t"], b["y"], None, F64)
+    kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=30011,
+              variant=_native.STANDARD, form=_native.FORM_SPLIT32, top_k=3)
+    plans = [device.SweepPlan(t, y, None, off, tile=tile, **kw) for tile in ((64, 16), (256, 8))]
+    assert [p.tile() for p in plans] == [(64, 16), (256, 8)]
+    for p in plans:
+        p.run()
+    torch.cuda.synchronize()
+    serial = [{k: v.clone() for k, v in p.results().items()} for p in plans]
+    errors, times = [], [[], []]
+
+    def worker(i):
+        try:
+            s = torch.cuda.Stream()
+            L.lsamd_set_timing(1)
+            for _ in range(6):
+                plans[i].run(s)
+                s.synchronize()
+                times[i].append(L.lsamd_last_sweep_kernel_ms())
+                out = plans[i].results()
+                for k in ("pgram", "index", "power", "period"):
+                    if not torch.equal(out[k], serial[i][k]):
+                        errors.append((i, k))
+            L.lsamd_set_timing(0)
+        except Exception as e:  # reported below
+            errors.append((i, repr(e)))
+
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errors, errors
+    assert all(ms > 0 for ts in times for ms in ts), times
+    assert_close(serial[0]["pgram"].cpu().numpy(), serial[1]["pgram"].cpu().numpy(), 2e-5, 1e-3, 4e-6)
 
 
 def test_device_segmentation_matches_host_runs():
