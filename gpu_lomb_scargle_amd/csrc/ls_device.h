@@ -22,8 +22,9 @@ struct SweepArgs {
     uint32_t slices;           // split forms: observation slices per (object, chunk), >= 1
     double *part_sums;         // slices > 1: per-slice FP64 sums [K*chunks][slices][NS][M][T]
     unsigned long long *queue; // persistent schedule: next item counter, else null
-    // plain
-    const void *t, *y, *w;
+    // plain (t, y, w); split forms: the caller's y and dy (DTYPE), read by
+    // the FP64 recomputation of ill-conditioned bins and small objects
+    const void *t, *y, *w, *dy;
     double fmin_r, fstep_r;    // DTYPE values stored in double
     // split
     const double *rec_t;
@@ -70,7 +71,9 @@ constexpr uint64_t kGroupMinItems = 65536;  // fewer items: one per fetch (e.g. 
 // split by observations (at least T observations per slice), up to about
 // this many items.
 constexpr uint64_t kSliceTargetItems = 2048;
-constexpr int kColfacFrags = 12;  // matrix-core tile: column-factor fragments per 16 observations
+// Matrix-core tile: column-factor fragments per 16 observations (standard:
+// y v and w; generalized: q^2 v, q (q y) v and q^2 w; head, tail, negated).
+constexpr int kColfacFragsStd = 12, kColfacFragsGen = 18;
 constexpr int kPeaks = 3;      // peaks tracked per object (top-1 and top-3)
 constexpr int kGen = 8;        // doubles per object of split-generalized constants
 constexpr double kInv2Pi = 0.159154943091895335768883763372514362;  // 1/(2 pi)

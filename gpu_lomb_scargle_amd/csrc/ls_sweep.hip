@@ -426,7 +426,8 @@ int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
     const TileEntry *te = find_tile(a->form, a->variant, a->dtype, tile.threads, tile.fpt);
     if (te && te->prep) {  // matrix-core tile: y scales and column-factor fragments
         w->obj_ys = take(4 * K);
-        w->colfac = take((size_t)16 * 64 * kColfacFrags * (n / 16 + K + 1));
+        const size_t frags = a->variant == LSAMD_GENERALIZED ? kColfacFragsGen : kColfacFragsStd;
+        w->colfac = take((size_t)16 * 64 * frags * (n / 16 + K + 1));
     }
     const size_t parts = (size_t)K * w->chunks * kPeaks;
     w->part_pow = take(8 * parts);
@@ -613,6 +614,7 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     sw.t = a->t;
     sw.y = plain_pre ? (const void *)(ws + w.rec_y) : a->y;
     sw.w = (!split && gen) ? (const void *)(ws + w.rec_w) : nullptr;
+    sw.dy = gen ? a->dy : nullptr;
     if (f64) {
         const double fmin = a->min_freq, fmax = a->max_freq;
         sw.fmin_r = fmin;

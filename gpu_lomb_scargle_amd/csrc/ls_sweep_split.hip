@@ -9,17 +9,273 @@ namespace lsamd {
 namespace {
 
 // ---------------------------------------------------------------------------
+// FP64 recomputation of ill-conditioned bins.  Every split-form power comes
+// from a 2x2 (standard) or centred 2x2 (generalized) least-squares system; its
+// relative error is the sums' relative error times a condition number that
+// the sums themselves give:
+//   standard:    k = (CC + SS)^2 / (4 D)                 (1 for balanced phases)
+//   generalized: k = (1 + C^2 + S^2) (CC + SS) / D      (centred CC, SS, CS;
+//                the 1 + C^2 + S^2 covers the centring CC = CCw - C^2)
+// with D = CC SS - CS^2.  The FP32 sums (split32) leave ~2.4e-7 k relative
+// (measured on the 1000-object generalized batch at the 1 cycle/day alias,
+// k ~ 110-230), the FP32-seeded FP64 sums (split) less.  Bins with
+// k > kCondMax (a few per 10^6 for nightly-sampled objects) and every bin of
+// objects of at most kSmallObs observations (5 points against the
+// generalized 3-parameter model can give k ~ 10^5) are computed again from
+// the caller's FP64 t, y, dy with FP64 phasors.
+// ---------------------------------------------------------------------------
+constexpr double kCondMax = 16.0;
+constexpr uint32_t kSmallObs = 16;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// sin and cos of 2 pi r for |r| <= 1/2 revolution: the octant q = rint(4 r)
+// is taken off exactly, and x = 2 pi (r - q/4), |x| <= pi/4, goes through the
+// Taylor series of sin to x^17 and of cos to x^16 (the first omitted terms
+// are below 5e-17 and 2e-18), in Horner form.
+__device__ __forceinline__ void sincos_rev64(double r, double &s, double &c) {
+    const double q = __builtin_rint(4.0 * r);
+    const double x = (r - 0.25 * q) * kTwoPi, x2 = x * x;
+    double ps = -1.0 / 355687428096000.0;  // -1/17!
+    ps = fma(ps, x2, 1.0 / 1307674368000.0);
+    ps = fma(ps, x2, -1.0 / 6227020800.0);
+    ps = fma(ps, x2, 1.0 / 39916800.0);
+    ps = fma(ps, x2, -1.0 / 362880.0);
+    ps = fma(ps, x2, 1.0 / 5040.0);
+    ps = fma(ps, x2, -1.0 / 120.0);
+    ps = fma(ps, x2, 1.0 / 6.0);
+    const double sx = fma(-ps * x2, x, x);  // x - x^3/3! + ... - x^17/17!... with alternating signs
+    double pc = 1.0 / 20922789888000.0;     // 1/16!
+    pc = fma(pc, x2, -1.0 / 87178291200.0);
+    pc = fma(pc, x2, 1.0 / 479001600.0);
+    pc = fma(pc, x2, -1.0 / 3628800.0);
+    pc = fma(pc, x2, 1.0 / 40320.0);
+    pc = fma(pc, x2, -1.0 / 720.0);
+    pc = fma(pc, x2, 1.0 / 24.0);
+    pc = fma(pc, x2, -0.5);
+    const double cx = fma(pc, x2, 1.0);
+    const int o = (int)q & 3;  // sin/cos of q pi/2 + x
+    s = o == 0 ? sx : o == 1 ? cx : o == 2 ? -sx : -cx;
+    c = o == 0 ? cx : o == 1 ? -sx : o == 2 ? -cx : sx;
+}
+
+// e^{2 pi i f t} in FP64: the phase reduced to [-1/2, 1/2] revolutions with
+// the product f t fused (exact before the subtraction).
+__device__ __forceinline__ void phasor64(double f, double t, double &c, double &s) {
+    const double n = __builtin_rint(f * t);
+    sincos_rev64(fma(f, t, -n), s, c);
+}
+
+// Power of the standard variant from its uncentred sums (SS = N - CC);
+// `ill` when its condition number exceeds kCondMax (or D <= 0, or NaN).
+__device__ __forceinline__ double std_power(double XC, double XS, double CC, double CS, double N, bool &ill) {
+    const double SS = N - CC, D = CC * SS - CS * CS;
+    ill = !((CC + SS) * (CC + SS) <= 4.0 * kCondMax * D) || !(D > 0.0);
+    return 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+}
+
+// Power of the generalized variant from sums over weights normalised to
+// sum 1 (C, S, CCw, SSw, CSw, YCw, YSw; Yb = weighted mean of the y in them,
+// YY the centred weighted sum of squares); `ill` as for std_power.
+__device__ __forceinline__ double gen_power(double C, double S, double CCw, double SSw, double CSw,
+                                            double YCw, double YSw, double Yb, double YY, bool &ill) {
+    const double YC = YCw - Yb * C, YS = YSw - Yb * S;
+    const double CC = CCw - C * C, SS = SSw - S * S, CS = CSw - C * S;
+    const double D = CC * SS - CS * CS;
+    ill = !((1.0 + C * C + S * S) * (CC + SS) <= kCondMax * D) || !(D > 0.0);
+    return (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
+}
+
+// FP64 power of bin kf of object obj (before the object's scale); every lane
+// of the wavefront calls it with the same (obj, kf) and gets the result.
+// Standard: raw y, unit weights.  Generalized: w = dy^-2 / W and y - ybar with
+// W and ybar from the statistics kernel (the power does not depend on them
+// beyond rounding: the weights are normalised and the mean floats).
+template <bool GEN, typename IN>
+__device__ double exact_power(const SweepArgs &a, uint32_t obj, uint32_t kf) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t b = a.offsets[obj], e = a.offsets[obj + 1];
+    const double f = a.fmin_rev + (double)kf * a.df_rev;
+    const IN *y = static_cast<const IN *>(a.y);
+    const IN *dy = static_cast<const IN *>(a.dy);
+    double iW = 1.0, ybar = 0.0;
+    if constexpr (GEN) {
+        iW = 1.0 / a.obj_gen[kGen * obj + 4];
+        ybar = a.obj_gen[kGen * obj + 3];
+    }
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0, s6 = 0.0, s7 = 0.0, s8 = 0.0;
+    for (uint64_t j = b + lane; j < e; j += 64) {
+        double c, sn;
+        phasor64(f, a.rec_t[j], c, sn);
+        if constexpr (GEN) {
+            const double dj = (double)dy[j], w = iW / (dj * dj), yc = (double)y[j] - ybar;
+            const double wc = w * c, wy = w * yc;
+            s0 += wc;
+            s1 = fma(w, sn, s1);
+            s2 = fma(wc, c, s2);
+            s3 = fma(wc, sn, s3);
+            s4 = fma(wy, c, s4);
+            s5 = fma(wy, sn, s5);
+            s6 += w;
+            s7 += wy;
+            s8 = fma(wy, yc, s8);
+        } else {
+            const double yj = (double)y[j];
+            s0 = fma(yj, c, s0);
+            s1 = fma(yj, sn, s1);
+            s2 = fma(c, c, s2);
+            s3 = fma(c, sn, s3);
+        }
+    }
+    bool ill;
+    if constexpr (GEN) {
+        const double W = wave_sum(s6), iWn = 1.0 / W;
+        const double C = wave_sum(s0) * iWn, S = wave_sum(s1) * iWn, CCw = wave_sum(s2) * iWn;
+        const double CSw = wave_sum(s3) * iWn, YCw = wave_sum(s4) * iWn, YSw = wave_sum(s5) * iWn;
+        const double Yb = wave_sum(s7) * iWn, YY = wave_sum(s8) * iWn - Yb * Yb;
+        return gen_power(C, S, CCw, 1.0 - CCw, CSw, YCw, YSw, Yb, YY, ill);
+    } else {
+        return std_power(wave_sum(s0), wave_sum(s1), wave_sum(s2), wave_sum(s3), (double)(e - b), ill);
+    }
+}
+
+// Recomputes, in FP64, the bins this wavefront's lanes flagged (bit m of a
+// lane's `flags` for its bin m, whose number bin_of(lane, m) gives), one bin
+// at a time with the whole wavefront.  One loop over the flagged (lane, bin)
+// pairs, so that exact_power is inlined once.
+template <bool GEN, typename IN, int M, typename BinOf>
+__device__ __forceinline__ void recompute_flagged(const SweepArgs &a, uint32_t obj, uint32_t flags, BinOf bin_of,
+                                                  double (&P)[M]) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t lanes = __ballot(flags != 0u);
+    while (lanes) {
+        const uint32_t src = (uint32_t)__builtin_ctzll(lanes);
+        const uint32_t f = (uint32_t)__shfl((int)flags, (int)src, 64);
+        const int m = __builtin_ctz(f);
+        const double pe = exact_power<GEN, IN>(a, obj, bin_of(src, m));
+        if (lane == src) {
+#pragma unroll
+            for (int q = 0; q < M; ++q) P[q] = q == m ? pe : P[q];
+            flags &= flags - 1u;
+        }
+        lanes = __ballot(flags != 0u);
+    }
+}
+
+// FP64 powers (before the object's scale) of a lane's 16 bins in the
+// matrix-core tile's layout, bin kbase + rc + 32 (8 (v / 4) + 4 kh + v % 4),
+// for an object of at most kSmallObs observations: per observation one FP64
+// phasor at the lane's first bin and FP64 rotations by 32 and 256 bins
+// (sums as exact_power's).
+template <bool GEN, typename IN>
+__device__ void small_object_powers(const SweepArgs &a, uint32_t obj, uint32_t kbase, uint32_t rc,
+                                    uint32_t kh, double (&P)[16]) {
+    constexpr int NS = GEN ? 6 : 4;  // generalized C, S, CCw, CSw, YCw, YSw; standard XC, XS, CC, CS
+    const uint64_t b = a.offsets[obj], e = a.offsets[obj + 1];
+    const double f0 = a.fmin_rev + (double)(kbase + rc + 128u * kh) * a.df_rev;
+    const double f32 = 32.0 * a.df_rev, f256 = 256.0 * a.df_rev;
+    const IN *y = static_cast<const IN *>(a.y);
+    const IN *dy = static_cast<const IN *>(a.dy);
+    double iW = 1.0, ybar = 0.0;
+    if constexpr (GEN) {
+        iW = 1.0 / a.obj_gen[kGen * obj + 4];
+        ybar = a.obj_gen[kGen * obj + 3];
+    }
+    double acc[NS][16];
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[q][v] = 0.0;
+    double W = 0.0, WY = 0.0, WYY = 0.0;
+    for (uint64_t j = b; j < e; ++j) {
+        const double t = a.rec_t[j];
+        double w = 1.0, yv;
+        if constexpr (GEN) {
+            const double dj = (double)dy[j];
+            w = iW / (dj * dj);
+            yv = (double)y[j] - ybar;
+            W += w;
+            WY = fma(w, yv, WY);
+            WYY = fma(w * yv, yv, WYY);
+        } else {
+            yv = (double)y[j];
+        }
+        double zc, zs, r1c, r1s, r8c, r8s;
+        phasor64(f0, t, zc, zs);
+        phasor64(f32, t, r1c, r1s);
+        phasor64(f256, t, r8c, r8s);
+        const double wy = w * yv;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            double c = zc, sn = zs;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int v = 4 * g + i;
+                if constexpr (GEN) {
+                    const double wc = w * c;
+                    acc[0][v] += wc;
+                    acc[1][v] = fma(w, sn, acc[1][v]);
+                    acc[2][v] = fma(wc, c, acc[2][v]);
+                    acc[3][v] = fma(wc, sn, acc[3][v]);
+                    acc[4][v] = fma(wy, c, acc[4][v]);
+                    acc[5][v] = fma(wy, sn, acc[5][v]);
+                } else {
+                    acc[0][v] = fma(yv, c, acc[0][v]);
+                    acc[1][v] = fma(yv, sn, acc[1][v]);
+                    acc[2][v] = fma(c, c, acc[2][v]);
+                    acc[3][v] = fma(c, sn, acc[3][v]);
+                }
+                if (i < 3) {
+                    const double cn = fma(c, r1c, -sn * r1s);
+                    sn = fma(sn, r1c, c * r1s);
+                    c = cn;
+                }
+            }
+            if (g < 3) {
+                const double cn = fma(zc, r8c, -zs * r8s);
+                zs = fma(zs, r8c, zc * r8s);
+                zc = cn;
+            }
+        }
+    }
+    const double N = (double)(e - b), iWn = 1.0 / W, Yb = WY * iWn, YY = WYY * iWn - Yb * Yb;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        bool ill;
+        if constexpr (GEN) {
+            const double CCw = acc[2][v] * iWn;
+            P[v] = gen_power(acc[0][v] * iWn, acc[1][v] * iWn, CCw, 1.0 - CCw, acc[3][v] * iWn, acc[4][v] * iWn,
+                             acc[5][v] * iWn, Yb, YY, ill);
+        } else {
+            P[v] = std_power(acc[0][v], acc[1][v], acc[2][v], acc[3][v], N, ill);
+        }
+    }
+}
+
+// Bin of result entry v of lane (rc, kh) in a matrix-core work item.
+__device__ __forceinline__ uint32_t mfma_bin(uint32_t kbase, uint32_t rc, uint32_t kh, int v) {
+    return kbase + 32u * (8u * (uint32_t)(v / 4) + 4u * kh + (uint32_t)(v % 4)) + rc;
+}
+
+// ---------------------------------------------------------------------------
 // Split forms: the powers of a thread's M bins from the object's FP64 sums
 // (tau-free power, normalisation, periodogram store, the chunk's peaks).
 //   standard:    XC, XS, CC, CS; SS = N - CC
 //   generalized: C, S, YC, YS, CC, [SS,] CS over the normalised weights; SS
 //                summed (SS_SUM, split32) or 1 - CC (split, whose unit-modulus
 //                seeds make the weights sum to 1 to FP64 rounding)
+// Bin m of thread tid: kbase + m T + tid, or with MFMA (sums of the matrix-core
+// tile, 64 threads x 16) its result layout, mfma_bin(kbase, tid % 32, tid / 32, m).
 // ---------------------------------------------------------------------------
-template <bool SS_SUM, bool GEN, typename OUT, int T, int M, int NS>
+template <bool SS_SUM, bool GEN, typename OUT, int T, int M, int NS, bool MFMA = false>
 __device__ __forceinline__ void split_epilogue(const SweepArgs &a, const double (&acc)[NS][M],
                                                uint32_t obj, uint32_t chunk,
                                                double (*s_pp)[kPeaks], uint32_t (*s_pi)[kPeaks]) {
+    static_assert(!MFMA || (T == 64 && M == 16), "matrix-core layout: 64 x 16");
     const uint32_t tid = threadIdx.x;
     const uint32_t kbase = chunk * (uint32_t)(T * M);
     const double scale = a.obj_scale[obj];
@@ -31,38 +287,37 @@ __device__ __forceinline__ void split_epilogue(const SweepArgs &a, const double 
         YY = a.obj_gen[kGen * obj + 2];
     }
     const double iW = 1.0 / Wf;  // one division per work item, not seven per bin
+    const bool small = nobs <= (double)kSmallObs;
+    double P[M];
+    uint32_t kf[M], flags = 0u;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        kf[m] = MFMA ? mfma_bin(kbase, tid & 31u, tid >> 5, m) : kbase + m * T + tid;
+        bool ill;
+        if constexpr (GEN) {
+            const double CCw = acc[4][m] * iW;
+            const double SSw = SS_SUM ? acc[5][m] * iW : 1.0 - CCw;
+            const double CSw = acc[SS_SUM ? 6 : 5][m] * iW;
+            P[m] = gen_power(acc[0][m] * iW, acc[1][m] * iW, CCw, SSw, CSw, acc[2][m] * iW, acc[3][m] * iW, Yb,
+                             YY, ill);
+        } else {
+            P[m] = std_power(acc[0][m], acc[1][m], acc[2][m], acc[3][m], nobs, ill);
+        }
+        if ((small || ill) && kf[m] < a.nf) flags |= 1u << m;
+    }
+    const uint32_t wbase = kbase + (tid & ~63u);  // bin of (lane 0, m = 0) of this wavefront
+    recompute_flagged<GEN, OUT, M>(a, obj, flags, [&](uint32_t src, int m) {
+        return MFMA ? mfma_bin(kbase, src & 31u, src >> 5, m) : wbase + (uint32_t)m * T + src;
+    }, P);
     OUT *pg = static_cast<OUT *>(a.pgram);
     Peaks L;
     peaks_init(L);
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-        const uint32_t kf = kbase + m * T + tid;
-        double P;
-        if constexpr (GEN) {
-            const double C = acc[0][m] * iW, S = acc[1][m] * iW;
-            const double YC = acc[2][m] * iW - Yb * C, YS = acc[3][m] * iW - Yb * S;
-            const double CCw = acc[4][m] * iW;
-            const double CC = CCw - C * C;
-            double SS, CS;
-            if constexpr (SS_SUM) {
-                SS = acc[5][m] * iW - S * S;
-                CS = acc[6][m] * iW - C * S;
-            } else {
-                SS = (1.0 - CCw) - S * S;
-                CS = acc[5][m] * iW - C * S;
-            }
-            const double D = CC * SS - CS * CS;
-            P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
-        } else {
-            const double XC = acc[0][m], XS = acc[1][m];
-            const double CC = acc[2][m], CS = acc[3][m], SS = nobs - CC;
-            const double D = CC * SS - CS * CS;
-            P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
-        }
-        const OUT pw = (OUT)(P * scale);
-        if (kf < a.nf) {
-            if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-            peaks_add(L, a.top_k, (double)pw, kf);
+        const OUT pw = (OUT)(P[m] * scale);
+        if (kf[m] < a.nf) {
+            if (pg) pg[(uint64_t)obj * a.nf + kf[m]] = pw;
+            peaks_add(L, a.top_k, (double)pw, kf[m]);
         }
     }
     if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
@@ -83,7 +338,9 @@ __device__ __forceinline__ void store_slice(const SweepArgs &a, const double (&a
 
 // Sums the observation slices of one (object, chunk) in slice order and
 // finishes it like the single-slice sweep.  One workgroup per (object, chunk).
-template <bool SS_SUM, bool GEN, typename OUT, int T, int M>
+// Slice sums of thread tid: [(q M + m) T + tid] (MFMA: the matrix-core
+// result layout, m = result entry v).
+template <bool SS_SUM, bool GEN, typename OUT, int T, int M, bool MFMA = false>
 __global__ __launch_bounds__(T) void k_combine_slices(SweepArgs a) {
     constexpr int NS = GEN ? (SS_SUM ? 7 : 6) : 4;
     __shared__ double s_pp[T / 64][kPeaks];
@@ -104,7 +361,7 @@ __global__ __launch_bounds__(T) void k_combine_slices(SweepArgs a) {
             for (int m = 0; m < M; ++m) acc[q][m] += src[(q * M + m) * T];
         src += NS * M * T;
     }
-    split_epilogue<SS_SUM, GEN, OUT, T, M, NS>(a, acc, obj, chunk, s_pp, s_pi);
+    split_epilogue<SS_SUM, GEN, OUT, T, M, NS, MFMA>(a, acc, obj, chunk, s_pp, s_pi);
 }
 
 // ---------------------------------------------------------------------------
@@ -486,20 +743,26 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
 //   Z2[a][b] = sum_j e^{4 pi i f_a t_j} e^{4 pi i b df t_j}          = C2 + i S2
 // with CC = (N + C2) / 2, CS = S2 / 2, SS = N - CC.  Each complex sum is four
 // real products over blocks of 16 observations, formed by
-// v_mfma_f32_32x32x16_f16 with FP32 accumulation.  Every factor, scaled by
-// 2^10, is split into an FP16 head h and a tail l = fp16(x - h); head x head
-// and the two cross products carry ~22 bits (the dropped tail x tail term is
-// 2^-22 of the product); each complex sum's head x head and cross products
-// accumulate in FP32 (apart, or in one sum for Z2 of the standard variant),
-// unscaled by 2^-20 in FP64.  The 2^10 scale keeps FP16 subnormal heads (which the VALU
-// conversions and the matrix core do not treat alike) to |x| < 2^-24; a
-// subnormal tail costs at most 2^-24 absolute.  Phases are reduced in FP64
-// and evaluated by the hardware sin/cos, as in the VALU split32 sweep.  The
-// factors of Z2 are divided by their squared modulus (hardware sin/cos leave
-// |z|^2 - 1 ~ 1e-7, and CC = (N + C2) / 2 assumes unit modulus).
+// v_mfma_f32_32x32x16_f16 with FP32 accumulation.  Every factor x, scaled by
+// 2^10, is split into an FP16 head h = fp16(x) and a tail l = fp16(x - h);
+// head x head and the two cross products carry ~22 bits (the dropped tail x
+// tail term is 2^-22 of the product); each complex sum's head x head and
+// cross products accumulate in FP32 (apart, or in one sum for Z2 of the
+// standard variant), unscaled by 2^-20 in FP64.  The head in the MFMA operand
+// must be the very value the tail was taken from (see split16).  An
+// FP16-subnormal head or tail (|x| <
+// 2^-14 after the scale, i.e. below 2^-24 of the unscaled factor) costs at
+// most that much absolute, whichever way the matrix core treats it.  Phases
+// are reduced in FP64 and evaluated by the hardware sin/cos, as in the VALU
+// split32 sweep.  The factors of Z2 are divided by their squared modulus
+// (hardware sin/cos leave |z|^2 - 1 ~ 1e-7, and CC = (N + C2) / 2 assumes
+// unit modulus).
 // The column factor depends only on the object: k_colfac writes its operand
 // fragments once per 16 observations, and y is scaled per object by a power
 // of two to max |y| in [0.5, 1) (FP16 range; undone exactly in the power).
+// Observations past the object's end have a zero column factor, and the
+// staging pads the row-side records with zeros, so partial blocks need no
+// masking.
 // Operand layout of 32x32x16 (checked on the device): lane l holds row (A) or
 // column (B) l % 32 and observations 8 (l / 32) .. +7 of the block; the
 // result holds column l % 32, rows 8 (v / 4) + 4 (l / 32) + v % 4, v < 16.
@@ -509,9 +772,24 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 constexpr float kFacScale = 1024.0f;  // 2^10 on each factor
 
-__device__ __forceinline__ void split16(float x, _Float16 &h, _Float16 &l) {
-    h = (_Float16)(fabsf(x) < 0x1p-14f ? 0.0f : x);  // no FP16 subnormal heads
-    l = (_Float16)(x - (float)h);                     // x - h is exact in FP32
+// Heads and tails of two factors x: h = fp16(x), l = fp16(x - h) (x - h is
+// exact in FP32).  The head must be one value.  When x is a product a b, the compiler may
+// round the exact product to FP16 in one step for the tail's head
+// (v_fma_mixlo_f16 h, a, b, 0) yet convert the FP32-rounded x for the packed
+// MFMA operand (v_cvt_pk_f16_f32); the two roundings differ by one FP16 ulp
+// in ~1 of 2^13 factors, which are then wrong by 2^-11 (seen as 4e-3 power
+// errors).  Here the two heads of a pair of factors come from one packed
+// conversion of their FP32 values (v_cvt_pk_f16_f32), and the tails are
+// taken against those very heads (the compiler may form x - h from the exact
+// product; that only makes h + l closer to it).  tests/test_split_rounding.py
+// checks the built kernels for the one-step product-to-FP16 form.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef float float2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void split16x2(float x0, float x1, half2_t &h, half2_t &l) {
+    h = __builtin_convertvector((float2_t){x0, x1}, half2_t);
+    const float2_t hf = __builtin_convertvector(h, float2_t);
+    l = __builtin_convertvector((float2_t){x0 - hf.x, x1 - hf.y}, half2_t);
 }
 
 // First column-factor slot of an object: the slots of object k are
@@ -519,69 +797,159 @@ __device__ __forceinline__ void split16(float x, _Float16 &h, _Float16 &l) {
 // next object's floor((beg + N)/16) + k + 1.
 __device__ __forceinline__ uint64_t colfac_slot(uint64_t beg, uint32_t obj) { return (beg >> 4) + obj; }
 
-// Wavefronts (object, y) with y < gridDim.y: the y scale (y = 0), then
-// kColfacFrags fragments per 16 observations for blocks y, y + gridDim.y, ...:
-// v_r, v_i, -v_i and w_r, w_i, -w_i (w = v^2), head and tail.  Observations
-// past the object's end have v = 0.
+// Wavefronts (object, y) with y < gridDim.y write the column-factor fragments
+// of 16-observation blocks y, y + gridDim.y, ... of the object, each factor
+// as head, tail (and negated head, tail where a product needs -f_i):
+//   standard:    y v (6 fragments: r, i, -i), w (6)
+//   generalized: q^2 v (6), q (q y_c) v (6), q^2 w (6)
+// with v = e^{2 pi i b df t} (b < 32 the column), w = v^2 / |v|^2, y the
+// object's y scaled by its power of two ys (wavefront y = 0 also stores ys),
+// q^2 the normalised weight and q y_c the records' weighted centred y.
+// Observations past the object's end have zero factors.
+template <bool GEN>
 __global__ __launch_bounds__(64) void k_colfac(SweepArgs a) {
+    constexpr int NF = GEN ? kColfacFragsGen : kColfacFragsStd;
     const uint32_t k = blockIdx.x, l = threadIdx.x;
     const uint64_t beg = a.offsets[k], end = a.offsets[k + 1];
     const uint32_t n = (uint32_t)(end - beg);
-    if (blockIdx.y == 0) {
-        float m = 0.0f;
-        for (uint64_t i = beg + l; i < end; i += 64) m = fmaxf(m, fabsf(a.rec_a[i].x));
+    float m = 0.0f;
+    for (uint64_t i = beg + l; i < end; i += 64) m = fmaxf(m, fabsf(a.rec_a[i].x));
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-        if (l == 0) {
-            int e = 0;
-            (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
-            a.obj_ys[k] = m > 0.0f ? ldexpf(1.0f, -e) : 1.0f;
-        }
-    }
+    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    int ex = 0;
+    (void)frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
+    const float ys = m > 0.0f ? ldexpf(1.0f, -ex) : 1.0f;
+    if (blockIdx.y == 0 && l == 0) a.obj_ys[k] = ys;
     const double fb = (double)(l & 31u) * a.df_rev;
-    uint4 *dst = a.colfac + colfac_slot(beg, k) * (uint64_t)(kColfacFrags * 64) + l;
+    uint4 *dst = a.colfac + colfac_slot(beg, k) * (uint64_t)(NF * 64) + l;
     for (uint32_t q = blockIdx.y; q * 16u < n; q += gridDim.y) {
-        half8_t f[kColfacFrags];
+        half8_t f[NF];
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            const uint32_t j = q * 16u + 8u * (l >> 5) + (uint32_t)v;
-            float cb = 0.0f, sb = 0.0f;
-            if (j < n) {
-                double pb = fb * a.rec_t[beg + j];
-                pb -= __builtin_rint(pb);
-                cb = __builtin_amdgcn_cosf((float)pb);
-                sb = __builtin_amdgcn_sinf((float)pb);
+        for (int v = 0; v < 8; v += 2) {
+            float x[NF / 3][2];  // the factors of observations v, v + 1: real, imaginary per sum
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const uint32_t j = q * 16u + 8u * (l >> 5) + (uint32_t)(v + e);
+                float cb = 0.0f, sb = 0.0f, yv = 0.0f, qv = 0.0f;
+                if (j < n) {
+                    double pb = fb * a.rec_t[beg + j];
+                    pb -= __builtin_rint(pb);
+                    cb = __builtin_amdgcn_cosf((float)pb);
+                    sb = __builtin_amdgcn_sinf((float)pb);
+                    const float4 r = a.rec_a[beg + j];
+                    yv = r.x * ys;  // exact: a power of two
+                    qv = r.y;
+                }
+                // w = v^2 / |v|^2 (1/n ~ 2 - n for n ~ 1), scaled
+                const float ssb = sb * sb, gb = kFacScale * (2.0f - fmaf(cb, cb, ssb));
+                const float wr = fmaf(cb, cb, -ssb) * gb, wi = 2.0f * cb * sb * gb;
+                const float c1 = cb * kFacScale, s1 = sb * kFacScale;  // exact: a power of two
+                if constexpr (GEN) {
+                    const float q2 = qv * qv, qy = qv * yv;
+                    x[0][e] = q2 * c1;
+                    x[1][e] = q2 * s1;
+                    x[2][e] = qy * c1;
+                    x[3][e] = qy * s1;
+                    x[4][e] = q2 * wr;
+                    x[5][e] = q2 * wi;
+                } else {
+                    x[0][e] = yv * c1;
+                    x[1][e] = yv * s1;
+                    x[2][e] = wr;
+                    x[3][e] = wi;
+                }
             }
-            _Float16 h, t;
-            split16(cb * kFacScale, h, t);
-            f[0][v] = h;
-            f[1][v] = t;
-            split16(sb * kFacScale, h, t);
-            f[2][v] = h;
-            f[3][v] = t;
-            f[4][v] = -h;
-            f[5][v] = -t;
-            // w = v^2 / |v|^2 (1/n ~ 2 - n for n ~ 1)
-            const float ccb = cb * cb, ssb = sb * sb, gb = kFacScale * (2.0f - (ccb + ssb));
-            split16((ccb - ssb) * gb, h, t);
-            f[6][v] = h;
-            f[7][v] = t;
-            split16(2.0f * cb * sb * gb, h, t);
-            f[8][v] = h;
-            f[9][v] = t;
-            f[10][v] = -h;
-            f[11][v] = -t;
+#pragma unroll
+            for (int p = 0; p < NF / 6; ++p) {  // each complex factor: r (h, l), i (h, l), -i (h, l)
+                half2_t h, t;
+                split16x2(x[2 * p][0], x[2 * p][1], h, t);
+                f[6 * p + 0][v] = h.x; f[6 * p + 0][v + 1] = h.y;
+                f[6 * p + 1][v] = t.x; f[6 * p + 1][v + 1] = t.y;
+                split16x2(x[2 * p + 1][0], x[2 * p + 1][1], h, t);
+                f[6 * p + 2][v] = h.x; f[6 * p + 2][v + 1] = h.y;
+                f[6 * p + 3][v] = t.x; f[6 * p + 3][v + 1] = t.y;
+                f[6 * p + 4][v] = -h.x; f[6 * p + 4][v + 1] = -h.y;
+                f[6 * p + 5][v] = -t.x; f[6 * p + 5][v + 1] = -t.y;
+            }
         }
 #pragma unroll
-        for (int i = 0; i < kColfacFrags; ++i) {
+        for (int i = 0; i < NF; ++i) {
             uint4 u;
             __builtin_memcpy(&u, &f[i], sizeof(u));
-            dst[(uint64_t)(q * kColfacFrags + i) * 64] = u;
+            dst[(uint64_t)(q * NF + i) * 64] = u;
         }
     }
 }
 
+// Row factors of a 16-observation block (t in LDS from s_t[j0]) for lane
+// (row rc, half kh): u = e^{2 pi i fa t} and u^2 / |u|^2 of the lane's 8
+// observations, scaled by 2^10, as FP16 heads and tails.
+struct RowFrags {
+    half8_t ur_h, ur_l, ui_h, ui_l, qr_h, qr_l, qi_h, qi_l;
+};
+__device__ __forceinline__ void row_factors(const double *s_t, uint32_t j0, uint32_t kh, double fa, RowFrags &R) {
+#pragma unroll
+    for (int v = 0; v < 8; v += 2) {
+        float x[4][2];  // u_r, u_i, (u^2/|u|^2)_r, (u^2/|u|^2)_i
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            double ph = fa * s_t[j0 + 8u * kh + (uint32_t)(v + e)];
+            ph -= __builtin_rint(ph);
+            const float c = __builtin_amdgcn_cosf((float)ph);
+            const float s = __builtin_amdgcn_sinf((float)ph);
+            x[0][e] = c * kFacScale;  // exact: a power of two
+            x[1][e] = s * kFacScale;
+            const float ss = s * s, g = kFacScale * (2.0f - fmaf(c, c, ss));
+            x[2][e] = fmaf(c, c, -ss) * g;
+            x[3][e] = (c + c) * s * g;
+        }
+        half2_t h, t;
+        split16x2(x[0][0], x[0][1], h, t);
+        R.ur_h[v] = h.x; R.ur_h[v + 1] = h.y;
+        R.ur_l[v] = t.x; R.ur_l[v + 1] = t.y;
+        split16x2(x[1][0], x[1][1], h, t);
+        R.ui_h[v] = h.x; R.ui_h[v + 1] = h.y;
+        R.ui_l[v] = t.x; R.ui_l[v + 1] = t.y;
+        split16x2(x[2][0], x[2][1], h, t);
+        R.qr_h[v] = h.x; R.qr_h[v + 1] = h.y;
+        R.qr_l[v] = t.x; R.qr_l[v + 1] = t.y;
+        split16x2(x[3][0], x[3][1], h, t);
+        R.qi_h[v] = h.x; R.qi_h[v + 1] = h.y;
+        R.qi_l[v] = t.x; R.qi_l[v + 1] = t.y;
+    }
+}
+
+// Loads the column-factor fragments of one 16-observation block.
+template <int NF>
+__device__ __forceinline__ void load_cols(const uint4 *cf, half8_t (&col)[NF]) {
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+        const uint4 u = cf[i * 64];
+        __builtin_memcpy(&col[i], &u, sizeof(u));
+    }
+}
+
 #define LS_MFMA(ACC, A, B) ACC = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B, ACC, 0, 0, 0)
+
+// Stores a lane's 16 powers (scaled by the object's scale) and merges the
+// item's peaks; kf ascends in v.
+template <typename OUT>
+__device__ __forceinline__ void mfma_store(const SweepArgs &a, uint32_t obj, uint32_t chunk, const uint32_t (&kf)[16],
+                                           const double (&P)[16], double scale, double (*s_pp)[kPeaks],
+                                           uint32_t (*s_pi)[kPeaks]) {
+    OUT *pg = static_cast<OUT *>(a.pgram);
+    Peaks L;
+    peaks_init(L);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const OUT pw = (OUT)(P[v] * scale);
+        if (kf[v] < a.nf) {
+            if (pg) pg[(uint64_t)obj * a.nf + kf[v]] = pw;
+            peaks_add(L, a.top_k, (double)pw, kf[v]);
+        }
+    }
+    if (a.top_k) write_partial<OUT, 64>(a, L, obj, chunk, s_pp, s_pi);
+}
 
 // Two waves per SIMD: six FP32 accumulators (96 registers; Z1 keeps its
 // cross products apart, where merging them doubles the rounding error of the
@@ -593,7 +961,6 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
     constexpr int T = 64, M = 16;  // 64 threads, 1024 bins per item
     constexpr uint32_t kFoldObs = 256;  // MFMA accumulation length
     __shared__ double s_t[T];
-    __shared__ float s_y[T];
     __shared__ float s_acc[4][16][T];  // sums of the 256-observation partials XC, XS, C2, S2
     __shared__ double s_pp[1][kPeaks];
     __shared__ uint32_t s_pi[1][kPeaks];
@@ -618,23 +985,35 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
             beg = b < oend ? b : oend;
             end = (beg + len) < oend ? beg + len : oend;
         }
-        const float ys = a.obj_ys[obj];
         const uint32_t kbase = chunk * (uint32_t)(T * M);
+        if (oend - obeg <= kSmallObs && a.slices == 1) {  // the whole item in FP64
+            double P[16];
+            uint32_t kf[16];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) kf[v] = mfma_bin(kbase, rc, kh, v);
+            small_object_powers<false, OUT>(a, obj, kbase, rc, kh, P);
+            mfma_store<OUT>(a, obj, chunk, kf, P, a.obj_scale[obj], s_pp, s_pi);
+            continue;
+        }
+        constexpr int NF = kColfacFragsStd;
+        const float ys = a.obj_ys[obj];
         const double fa = a.fmin_rev + (double)(kbase + 32u * rc) * a.df_rev;  // row rc, column 0
-        const uint4 *cf = a.colfac +
-                          (colfac_slot(obeg, obj) + ((beg - obeg) >> 4)) * (uint64_t)(kColfacFrags * 64) + l;
+        const uint4 *cf = a.colfac + (colfac_slot(obeg, obj) + ((beg - obeg) >> 4)) * (uint64_t)(NF * 64) + l;
         f32x16_t z1r = {}, z1r_x = {}, z1i = {}, z1i_x = {};  // Z1: head x head, cross products
         f32x16_t z2r = {}, z2i = {};
         // after every kFoldObs observations the accumulators move into FP32
-        // sums in LDS (each lane its own entries, so no barrier)
+        // sums in LDS (each lane its own entries, so no barrier); entry (q, v)
+        // of this lane is acc_l[(16 q + v) T]
+        float *const acc_l = &s_acc[0][0][l];
         bool folded = false;
         auto fold = [&]() {
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                s_acc[0][v][l] = (folded ? s_acc[0][v][l] : 0.0f) + (z1r[v] + z1r_x[v]);
-                s_acc[1][v][l] = (folded ? s_acc[1][v][l] : 0.0f) + (z1i[v] + z1i_x[v]);
-                s_acc[2][v][l] = (folded ? s_acc[2][v][l] : 0.0f) + z2r[v];
-                s_acc[3][v][l] = (folded ? s_acc[3][v][l] : 0.0f) + z2i[v];
+                float *e = acc_l + v * T;
+                e[0 * 16 * T] = (folded ? e[0 * 16 * T] : 0.0f) + (z1r[v] + z1r_x[v]);
+                e[1 * 16 * T] = (folded ? e[1 * 16 * T] : 0.0f) + (z1i[v] + z1i_x[v]);
+                e[2 * 16 * T] = (folded ? e[2 * 16 * T] : 0.0f) + z2r[v];
+                e[3 * 16 * T] = (folded ? e[3 * 16 * T] : 0.0f) + z2i[v];
             }
             z1r = f32x16_t{};
             z1r_x = f32x16_t{};
@@ -649,72 +1028,41 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
         for (uint64_t b0 = g0; b0 < gend; b0 += T) {
             const uint32_t cnt = (uint32_t)((gend - b0) < (uint64_t)T ? (gend - b0) : (uint64_t)T);
             __syncthreads();  // also orders the previous item's peak merge before this one's
-            if (l < cnt) {
-                s_t[l] = a.rec_t[b0 + l];
-                s_y[l] = a.rec_a[b0 + l].x * (ys * kFacScale);
-            }
+            // past cnt (the object's end) t = 0: finite row factors against
+            // zero column factors
+            s_t[l] = l < cnt ? a.rec_t[b0 + l] : 0.0;
             __syncthreads();
-            for (uint32_t j0 = 0; j0 < cnt; j0 += 16, cf += kColfacFrags * 64) {
-                half8_t col[kColfacFrags];
-#pragma unroll
-                for (int i = 0; i < kColfacFrags; ++i) {
-                    const uint4 u = cf[i * 64];
-                    __builtin_memcpy(&col[i], &u, sizeof(u));
-                }
-                half8_t ar_h, ar_l, ai_h, ai_l, qr_h, qr_l, qi_h, qi_l;  // y u and u^2
-#pragma unroll
-                for (int v = 0; v < 8; ++v) {
-                    const uint32_t j = j0 + 8u * kh + (uint32_t)v;  // < T: j0 <= T - 16
-                    const bool ok = j < cnt;
-                    const double tj = s_t[j];
-                    const float yj = ok ? s_y[j] : 0.0f;
-                    double ph = fa * tj;
-                    ph -= __builtin_rint(ph);
-                    float c = __builtin_amdgcn_cosf((float)ph);
-                    float s = __builtin_amdgcn_sinf((float)ph);
-                    c = ok ? c : 0.0f;
-                    s = ok ? s : 0.0f;
-                    _Float16 h, t;
-                    split16(yj * c, h, t);
-                    ar_h[v] = h;
-                    ar_l[v] = t;
-                    split16(yj * s, h, t);
-                    ai_h[v] = h;
-                    ai_l[v] = t;
-                    const float cc = c * c, ss = s * s, g = kFacScale * (2.0f - (cc + ss));  // u^2/|u|^2
-                    split16((cc - ss) * g, h, t);
-                    qr_h[v] = h;
-                    qr_l[v] = t;
-                    split16(2.0f * c * s * g, h, t);
-                    qi_h[v] = h;
-                    qi_l[v] = t;
-                }
-                // Re Z1 = (y u)_r v_r - (y u)_i v_i,  Im Z1 = (y u)_r v_i + (y u)_i v_r
-                LS_MFMA(z1r, ar_h, col[0]);
-                LS_MFMA(z1r_x, ar_h, col[1]);
-                LS_MFMA(z1r_x, ar_l, col[0]);
-                LS_MFMA(z1r, ai_h, col[4]);
-                LS_MFMA(z1r_x, ai_h, col[5]);
-                LS_MFMA(z1r_x, ai_l, col[4]);
-                LS_MFMA(z1i, ar_h, col[2]);
-                LS_MFMA(z1i_x, ar_h, col[3]);
-                LS_MFMA(z1i_x, ar_l, col[2]);
-                LS_MFMA(z1i, ai_h, col[0]);
-                LS_MFMA(z1i_x, ai_h, col[1]);
-                LS_MFMA(z1i_x, ai_l, col[0]);
-                // the same for Z2 with u^2 and w
-                LS_MFMA(z2r, qr_h, col[6]);
-                LS_MFMA(z2r, qr_h, col[7]);
-                LS_MFMA(z2r, qr_l, col[6]);
-                LS_MFMA(z2r, qi_h, col[10]);
-                LS_MFMA(z2r, qi_h, col[11]);
-                LS_MFMA(z2r, qi_l, col[10]);
-                LS_MFMA(z2i, qr_h, col[8]);
-                LS_MFMA(z2i, qr_h, col[9]);
-                LS_MFMA(z2i, qr_l, col[8]);
-                LS_MFMA(z2i, qi_h, col[6]);
-                LS_MFMA(z2i, qi_h, col[7]);
-                LS_MFMA(z2i, qi_l, col[6]);
+            for (uint32_t j0 = 0; j0 < cnt; j0 += 16, cf += NF * 64) {
+                half8_t col[NF];
+                load_cols<NF>(cf, col);
+                RowFrags R;
+                row_factors(s_t, j0, kh, fa, R);
+                // Re Z1 = u_r (y v)_r - u_i (y v)_i,  Im Z1 = u_r (y v)_i + u_i (y v)_r
+                LS_MFMA(z1r, R.ur_h, col[0]);
+                LS_MFMA(z1r_x, R.ur_h, col[1]);
+                LS_MFMA(z1r_x, R.ur_l, col[0]);
+                LS_MFMA(z1r, R.ui_h, col[4]);
+                LS_MFMA(z1r_x, R.ui_h, col[5]);
+                LS_MFMA(z1r_x, R.ui_l, col[4]);
+                LS_MFMA(z1i, R.ur_h, col[2]);
+                LS_MFMA(z1i_x, R.ur_h, col[3]);
+                LS_MFMA(z1i_x, R.ur_l, col[2]);
+                LS_MFMA(z1i, R.ui_h, col[0]);
+                LS_MFMA(z1i_x, R.ui_h, col[1]);
+                LS_MFMA(z1i_x, R.ui_l, col[0]);
+                // the same for Z2 with u^2/|u|^2 and w
+                LS_MFMA(z2r, R.qr_h, col[6]);
+                LS_MFMA(z2r, R.qr_h, col[7]);
+                LS_MFMA(z2r, R.qr_l, col[6]);
+                LS_MFMA(z2r, R.qi_h, col[10]);
+                LS_MFMA(z2r, R.qi_h, col[11]);
+                LS_MFMA(z2r, R.qi_l, col[10]);
+                LS_MFMA(z2i, R.qr_h, col[8]);
+                LS_MFMA(z2i, R.qr_h, col[9]);
+                LS_MFMA(z2i, R.qr_l, col[8]);
+                LS_MFMA(z2i, R.qi_h, col[6]);
+                LS_MFMA(z2i, R.qi_h, col[7]);
+                LS_MFMA(z2i, R.qi_l, col[6]);
             }
         }
         if (gend < end) fold();
@@ -726,10 +1074,11 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
             C2 = (double)z2r[v];
             S2 = (double)z2i[v];
             if (folded) {
-                XC += (double)s_acc[0][v][l];
-                XS += (double)s_acc[1][v][l];
-                C2 += (double)s_acc[2][v][l];
-                S2 += (double)s_acc[3][v][l];
+                const float *e = acc_l + v * T;
+                XC += (double)e[0 * 16 * T];
+                XS += (double)e[1 * 16 * T];
+                C2 += (double)e[2 * 16 * T];
+                S2 += (double)e[3 * 16 * T];
             }
             XC *= kUnscale;
             XS *= kUnscale;
@@ -738,45 +1087,37 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
         };
         const double iys = 1.0 / (double)ys;  // a power of two
         if (a.slices > 1) {
-            // the slice's FP64 sums in the layout k_combine_slices reads:
-            // XC, XS, CC, CS of bin m T + tid at [(q M + m) T + tid]
+            // the slice's FP64 sums XC, XS, CC, CS of result entry v at
+            // [(q M + v) T + l], the matrix-core layout k_combine_slices reads
             double *dst = a.part_sums + (((uint64_t)obj * a.chunks + chunk) * a.slices + slice) *
-                                            (uint64_t)(4 * M * T);
+                                            (uint64_t)(4 * M * T) + l;
             const double ns = (double)(end - beg);
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const uint32_t row = 8u * (uint32_t)(v / 4) + 4u * kh + (uint32_t)(v % 4);
-                const uint32_t bl = 32u * row + rc, m = bl / T, tt = bl % T;
                 double XC, XS, C2, S2;
                 sums(v, XC, XS, C2, S2);
-                dst[(0 * M + m) * T + tt] = XC * iys;
-                dst[(1 * M + m) * T + tt] = XS * iys;
-                dst[(2 * M + m) * T + tt] = 0.5 * (ns + C2);
-                dst[(3 * M + m) * T + tt] = 0.5 * S2;
+                dst[(0 * M + v) * T] = XC * iys;
+                dst[(1 * M + v) * T] = XS * iys;
+                dst[(2 * M + v) * T] = 0.5 * (ns + C2);
+                dst[(3 * M + v) * T] = 0.5 * S2;
             }
             continue;
         }
-        const double scale = a.obj_scale[obj] * iys * iys;
         const double nobs = (double)(oend - obeg);
-        OUT *pg = static_cast<OUT *>(a.pgram);
-        Peaks L;
-        peaks_init(L);
+        double P[16];
+        uint32_t kf[16], flags = 0u;
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            const uint32_t row = 8u * (uint32_t)(v / 4) + 4u * kh + (uint32_t)(v % 4);
-            const uint32_t kf = kbase + 32u * row + rc;  // ascending in v
+            kf[v] = mfma_bin(kbase, rc, kh, v);
             double XC, XS, C2, S2;
+            bool ill;
             sums(v, XC, XS, C2, S2);
-            const double CC = 0.5 * (nobs + C2), SS = nobs - CC, CS = 0.5 * S2;
-            const double D = CC * SS - CS * CS;
-            const double P = 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
-            const OUT pw = (OUT)(P * scale);
-            if (kf < a.nf) {
-                if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-                peaks_add(L, a.top_k, (double)pw, kf);
-            }
+            P[v] = std_power(XC * iys, XS * iys, 0.5 * (nobs + C2), 0.5 * S2, nobs, ill);
+            if (ill && kf[v] < a.nf) flags |= 1u << v;
         }
-        if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+        recompute_flagged<false, OUT, 16>(
+            a, obj, flags, [&](uint32_t src, int v) { return mfma_bin(kbase, src & 31u, src >> 5, v); }, P);
+        mfma_store<OUT>(a, obj, chunk, kf, P, a.obj_scale[obj], s_pp, s_pi);
     }
 }
 
@@ -786,17 +1127,22 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
 //   Zw = sum q^2 e^{i phi},  Zy = sum q (q y_c) e^{i phi},  Z2 = sum q^2 e^{2 i phi}
 // give the seven weighted sums of the generalized power: C, S = Zw;
 // YC, YS = Zy; CC, SS = (Wf +- Re Z2) / 2; CS = Im Z2 / 2, with Wf = sum q^2.
-// 36 MFMAs and 12 accumulators (head x head, cross products) per 16
-// observations.  The fold keeps Zy in FP64 and the weight-only Zw and Z2 in
-// FP32 (34 KiB of LDS: four workgroups per CU, one per SIMD).
+// 36 MFMAs per 16 observations.
+// Zy sets the precision of weak bins (P ~ 1e-3 P_max): every 16-observation
+// block forms it in zeroed FP32 accumulators (head x head and cross products
+// apart), so that their rounding errors scale with the block's sum rather
+// than the running one, and adds it to FP64 running sums in registers.  The
+// weight-only Zw and Z2 (head x head and cross products apart) accumulate
+// across 256 observations and fold into FP32 sums in LDS; they feed the
+// centred CC - C^2, SS - S^2, CS - C S, which cancel at aliasing frequencies,
+// and those bins exceed the condition bound and are recomputed in FP64.
+// One wavefront per SIMD (more than 256 registers per lane); 16 KiB of LDS.
 template <typename OUT>
 __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
     constexpr int T = 64, M = 16;
     constexpr uint32_t kFoldObs = 256;
     __shared__ double s_t[T];
-    __shared__ float s_q[T], s_y[T];
-    __shared__ double s_accy[2][16][T];  // FP64 Re/Im of Zy of a lane's 16 bins
-    __shared__ float s_accw[4][16][T];   // FP32 Re/Im of Zw and Z2 (weights only)
+    __shared__ float s_accw[4][16][T];  // FP32 Re/Im of Zw and Z2 of a lane's 16 bins
     __shared__ double s_pp[1][kPeaks];
     __shared__ uint32_t s_pi[1][kPeaks];
     __shared__ uint64_t s_item;
@@ -819,32 +1165,41 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
             beg = b < oend ? b : oend;
             end = (beg + len) < oend ? beg + len : oend;
         }
-        const float ys = a.obj_ys[obj];
         const uint32_t kbase = chunk * (uint32_t)(T * M);
+        if (oend - obeg <= kSmallObs && a.slices == 1) {  // the whole item in FP64
+            double P[16];
+            uint32_t kf[16];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) kf[v] = mfma_bin(kbase, rc, kh, v);
+            small_object_powers<true, OUT>(a, obj, kbase, rc, kh, P);
+            mfma_store<OUT>(a, obj, chunk, kf, P, a.obj_scale[obj], s_pp, s_pi);
+            continue;
+        }
+        constexpr int NF = kColfacFragsGen;
+        const float ys = a.obj_ys[obj];
         const double fa = a.fmin_rev + (double)(kbase + 32u * rc) * a.df_rev;
-        const uint4 *cf = a.colfac +
-                          (colfac_slot(obeg, obj) + ((beg - obeg) >> 4)) * (uint64_t)(kColfacFrags * 64) + l;
-        f32x16_t zw[4] = {}, zy[4] = {}, z2[4] = {};  // {Re hh, Re x, Im hh, Im x}
-        double wq = 0.0;                             // this lane's share of sum q^2 (slices)
+        const uint4 *cf = a.colfac + (colfac_slot(obeg, obj) + ((beg - obeg) >> 4)) * (uint64_t)(NF * 64) + l;
+        f32x16_t zw[4] = {}, z2[4] = {};  // {Re hh, Re x, Im hh, Im x} since the last fold
+        double ry[2][16];                  // FP64 running Re, Im of Zy
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) ry[q][v] = 0.0;
+        // entry (q, v) of this lane's LDS sums: acc_l[(16 q + v) T]
+        float *const acc_l = &s_accw[0][0][l];
         bool folded = false;
         auto fold = [&]() {
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                float pw[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) pw[q] = folded ? s_accw[q][v][l] : 0.0f;
-                const double py0 = folded ? s_accy[0][v][l] : 0.0, py1 = folded ? s_accy[1][v][l] : 0.0;
-                s_accw[0][v][l] = pw[0] + (zw[0][v] + zw[1][v]);
-                s_accw[1][v][l] = pw[1] + (zw[2][v] + zw[3][v]);
-                s_accy[0][v][l] = py0 + (double)zy[0][v] + (double)zy[1][v];
-                s_accy[1][v][l] = py1 + (double)zy[2][v] + (double)zy[3][v];
-                s_accw[2][v][l] = pw[2] + (z2[0][v] + z2[1][v]);
-                s_accw[3][v][l] = pw[3] + (z2[2][v] + z2[3][v]);
+                float *e = acc_l + v * T;
+                e[0 * 16 * T] = (folded ? e[0 * 16 * T] : 0.0f) + (zw[0][v] + zw[1][v]);
+                e[1 * 16 * T] = (folded ? e[1 * 16 * T] : 0.0f) + (zw[2][v] + zw[3][v]);
+                e[2 * 16 * T] = (folded ? e[2 * 16 * T] : 0.0f) + (z2[0][v] + z2[1][v]);
+                e[3 * 16 * T] = (folded ? e[3 * 16 * T] : 0.0f) + (z2[2][v] + z2[3][v]);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 zw[i] = f32x16_t{};
-                zy[i] = f32x16_t{};
                 z2[i] = f32x16_t{};
             }
             folded = true;
@@ -854,94 +1209,62 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
         for (uint64_t b0 = g0; b0 < gend; b0 += T) {
             const uint32_t cnt = (uint32_t)((gend - b0) < (uint64_t)T ? (gend - b0) : (uint64_t)T);
             __syncthreads();  // also orders the previous item's peak merge before this one's
-            if (l < cnt) {
-                const float4 r = a.rec_a[b0 + l];
-                s_t[l] = a.rec_t[b0 + l];
-                s_q[l] = r.y;
-                s_y[l] = r.x * (ys * kFacScale);
-                wq += (double)(r.y * r.y);
-            }
+            // past cnt (the object's end) t = 0: finite row factors against
+            // zero column factors
+            s_t[l] = l < cnt ? a.rec_t[b0 + l] : 0.0;
             __syncthreads();
-            for (uint32_t j0 = 0; j0 < cnt; j0 += 16, cf += kColfacFrags * 64) {
-                half8_t col[kColfacFrags];
+            for (uint32_t j0 = 0; j0 < cnt; j0 += 16, cf += NF * 64) {
+                half8_t col[NF];
+                load_cols<NF>(cf, col);
+                RowFrags R;
+                row_factors(s_t, j0, kh, fa, R);
+                // Re Z = u_r f_r - u_i f_i,  Im Z = u_r f_i + u_i f_r for the
+                // column factors f = q^2 v (col 0..5) of Zw, q (q y) v (6..11)
+                // of Zy, and, with u^2/|u|^2 for u, q^2 w (12..17) of Z2;
+                // head x head ([0], [2]) and cross products ([1], [3]) apart.
+                LS_MFMA(zw[0], R.ur_h, col[0]);
+                LS_MFMA(zw[1], R.ur_h, col[1]);
+                LS_MFMA(zw[1], R.ur_l, col[0]);
+                LS_MFMA(zw[0], R.ui_h, col[4]);
+                LS_MFMA(zw[1], R.ui_h, col[5]);
+                LS_MFMA(zw[1], R.ui_l, col[4]);
+                LS_MFMA(zw[2], R.ur_h, col[2]);
+                LS_MFMA(zw[3], R.ur_h, col[3]);
+                LS_MFMA(zw[3], R.ur_l, col[2]);
+                LS_MFMA(zw[2], R.ui_h, col[0]);
+                LS_MFMA(zw[3], R.ui_h, col[1]);
+                LS_MFMA(zw[3], R.ui_l, col[0]);
+                LS_MFMA(z2[0], R.qr_h, col[12]);
+                LS_MFMA(z2[1], R.qr_h, col[13]);
+                LS_MFMA(z2[1], R.qr_l, col[12]);
+                LS_MFMA(z2[0], R.qi_h, col[16]);
+                LS_MFMA(z2[1], R.qi_h, col[17]);
+                LS_MFMA(z2[1], R.qi_l, col[16]);
+                LS_MFMA(z2[2], R.qr_h, col[14]);
+                LS_MFMA(z2[3], R.qr_h, col[15]);
+                LS_MFMA(z2[3], R.qr_l, col[14]);
+                LS_MFMA(z2[2], R.qi_h, col[12]);
+                LS_MFMA(z2[3], R.qi_h, col[13]);
+                LS_MFMA(z2[3], R.qi_l, col[12]);
+                // the block's Zy from zero, cross products first
+                f32x16_t zy[4] = {};
+                LS_MFMA(zy[1], R.ur_h, col[7]);
+                LS_MFMA(zy[1], R.ur_l, col[6]);
+                LS_MFMA(zy[1], R.ui_h, col[11]);
+                LS_MFMA(zy[1], R.ui_l, col[10]);
+                LS_MFMA(zy[0], R.ur_h, col[6]);
+                LS_MFMA(zy[0], R.ui_h, col[10]);
+                LS_MFMA(zy[3], R.ur_h, col[9]);
+                LS_MFMA(zy[3], R.ur_l, col[8]);
+                LS_MFMA(zy[3], R.ui_h, col[7]);
+                LS_MFMA(zy[3], R.ui_l, col[6]);
+                LS_MFMA(zy[2], R.ur_h, col[8]);
+                LS_MFMA(zy[2], R.ui_h, col[6]);
 #pragma unroll
-                for (int i = 0; i < kColfacFrags; ++i) {
-                    const uint4 u = cf[i * 64];
-                    __builtin_memcpy(&col[i], &u, sizeof(u));
+                for (int v = 0; v < 16; ++v) {
+                    ry[0][v] += (double)(zy[0][v] + zy[1][v]);
+                    ry[1][v] += (double)(zy[2][v] + zy[3][v]);
                 }
-                half8_t wr_h, wr_l, wi_h, wi_l, yr_h, yr_l, yi_h, yi_l, qr_h, qr_l, qi_h, qi_l;
-#pragma unroll
-                for (int v = 0; v < 8; ++v) {
-                    const uint32_t j = j0 + 8u * kh + (uint32_t)v;  // < T: j0 <= T - 16
-                    const bool ok = j < cnt;
-                    const double tj = s_t[j];
-                    const float qj = ok ? s_q[j] : 0.0f, yj = ok ? s_y[j] : 0.0f;
-                    double ph = fa * tj;
-                    ph -= __builtin_rint(ph);
-                    float c = __builtin_amdgcn_cosf((float)ph);
-                    float s = __builtin_amdgcn_sinf((float)ph);
-                    c = ok ? c : 0.0f;
-                    s = ok ? s : 0.0f;
-                    const float q2 = qj * qj * kFacScale, qy = qj * yj;
-                    _Float16 h, t;
-                    split16(q2 * c, h, t);
-                    wr_h[v] = h;
-                    wr_l[v] = t;
-                    split16(q2 * s, h, t);
-                    wi_h[v] = h;
-                    wi_l[v] = t;
-                    split16(qy * c, h, t);
-                    yr_h[v] = h;
-                    yr_l[v] = t;
-                    split16(qy * s, h, t);
-                    yi_h[v] = h;
-                    yi_l[v] = t;
-                    const float cc = c * c, ss = s * s, g = q2 * (2.0f - (cc + ss));  // q^2 u^2/|u|^2
-                    split16((cc - ss) * g, h, t);
-                    qr_h[v] = h;
-                    qr_l[v] = t;
-                    split16(2.0f * c * s * g, h, t);
-                    qi_h[v] = h;
-                    qi_l[v] = t;
-                }
-                // Re Z = p_r f_r - p_i f_i,  Im Z = p_r f_i + p_i f_r; column
-                // fragments f = v (col 0..5) for Zw, Zy and w = v^2 (col 6..11) for Z2
-                LS_MFMA(zw[0], wr_h, col[0]);
-                LS_MFMA(zw[1], wr_h, col[1]);
-                LS_MFMA(zw[1], wr_l, col[0]);
-                LS_MFMA(zw[0], wi_h, col[4]);
-                LS_MFMA(zw[1], wi_h, col[5]);
-                LS_MFMA(zw[1], wi_l, col[4]);
-                LS_MFMA(zw[2], wr_h, col[2]);
-                LS_MFMA(zw[3], wr_h, col[3]);
-                LS_MFMA(zw[3], wr_l, col[2]);
-                LS_MFMA(zw[2], wi_h, col[0]);
-                LS_MFMA(zw[3], wi_h, col[1]);
-                LS_MFMA(zw[3], wi_l, col[0]);
-                LS_MFMA(zy[0], yr_h, col[0]);
-                LS_MFMA(zy[1], yr_h, col[1]);
-                LS_MFMA(zy[1], yr_l, col[0]);
-                LS_MFMA(zy[0], yi_h, col[4]);
-                LS_MFMA(zy[1], yi_h, col[5]);
-                LS_MFMA(zy[1], yi_l, col[4]);
-                LS_MFMA(zy[2], yr_h, col[2]);
-                LS_MFMA(zy[3], yr_h, col[3]);
-                LS_MFMA(zy[3], yr_l, col[2]);
-                LS_MFMA(zy[2], yi_h, col[0]);
-                LS_MFMA(zy[3], yi_h, col[1]);
-                LS_MFMA(zy[3], yi_l, col[0]);
-                LS_MFMA(z2[0], qr_h, col[6]);
-                LS_MFMA(z2[1], qr_h, col[7]);
-                LS_MFMA(z2[1], qr_l, col[6]);
-                LS_MFMA(z2[0], qi_h, col[10]);
-                LS_MFMA(z2[1], qi_h, col[11]);
-                LS_MFMA(z2[1], qi_l, col[10]);
-                LS_MFMA(z2[2], qr_h, col[8]);
-                LS_MFMA(z2[3], qr_h, col[9]);
-                LS_MFMA(z2[3], qr_l, col[8]);
-                LS_MFMA(z2[2], qi_h, col[6]);
-                LS_MFMA(z2[3], qi_h, col[7]);
-                LS_MFMA(z2[3], qi_l, col[6]);
             }
         }
         if (gend < end) fold();
@@ -950,72 +1273,64 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
         auto sums = [&](int v, double (&z)[6]) {
             z[0] = (double)zw[0][v] + (double)zw[1][v];
             z[1] = (double)zw[2][v] + (double)zw[3][v];
-            z[2] = (double)zy[0][v] + (double)zy[1][v];
-            z[3] = (double)zy[2][v] + (double)zy[3][v];
+            z[2] = ry[0][v];
+            z[3] = ry[1][v];
             z[4] = (double)z2[0][v] + (double)z2[1][v];
             z[5] = (double)z2[2][v] + (double)z2[3][v];
             if (folded) {
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    z[q] += s_accw[q][v][l];
-                    z[2 + q] += s_accy[q][v][l];
-                    z[4 + q] += s_accw[2 + q][v][l];
-                }
+                const float *e = acc_l + v * T;
+                z[0] += (double)e[0 * 16 * T];
+                z[1] += (double)e[1 * 16 * T];
+                z[4] += (double)e[2 * 16 * T];
+                z[5] += (double)e[3 * 16 * T];
             }
             const double iys = 1.0 / (double)ys;
 #pragma unroll
             for (int q = 0; q < 6; ++q) z[q] *= kUnscale * (q == 2 || q == 3 ? iys : 1.0);
         };
         if (a.slices > 1) {
-            // the slice's seven FP64 sums in the layout k_combine_slices reads,
-            // with the slice's weight sum for CC and SS
-            double wsum = wq;
+            // the slice's seven FP64 sums in the matrix-core layout
+            // k_combine_slices reads, with the slice's weight sum for CC and SS
+            double wsum = 0.0;
+            for (uint64_t i = beg + l; i < end; i += T) {
+                const float q = a.rec_a[i].y;
+                wsum += (double)(q * q);
+            }
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) wsum += __shfl_xor(wsum, off, 64);
             double *dst = a.part_sums + (((uint64_t)obj * a.chunks + chunk) * a.slices + slice) *
-                                            (uint64_t)(7 * M * T);
+                                            (uint64_t)(7 * M * T) + l;
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const uint32_t row = 8u * (uint32_t)(v / 4) + 4u * kh + (uint32_t)(v % 4);
-                const uint32_t bl = 32u * row + rc, m = bl / T, tt = bl % T;
                 double z[6];
                 sums(v, z);
-                dst[(0 * M + m) * T + tt] = z[0];
-                dst[(1 * M + m) * T + tt] = z[1];
-                dst[(2 * M + m) * T + tt] = z[2];
-                dst[(3 * M + m) * T + tt] = z[3];
-                dst[(4 * M + m) * T + tt] = 0.5 * (wsum + z[4]);
-                dst[(5 * M + m) * T + tt] = 0.5 * (wsum - z[4]);
-                dst[(6 * M + m) * T + tt] = 0.5 * z[5];
+                dst[(0 * M + v) * T] = z[0];
+                dst[(1 * M + v) * T] = z[1];
+                dst[(2 * M + v) * T] = z[2];
+                dst[(3 * M + v) * T] = z[3];
+                dst[(4 * M + v) * T] = 0.5 * (wsum + z[4]);
+                dst[(5 * M + v) * T] = 0.5 * (wsum - z[4]);
+                dst[(6 * M + v) * T] = 0.5 * z[5];
             }
             continue;
         }
-        const double scale = a.obj_scale[obj];
         const double Wf = a.obj_gen[kGen * obj + 0], Yb = a.obj_gen[kGen * obj + 1];
         const double YY = a.obj_gen[kGen * obj + 2], iW = 1.0 / Wf;
-        OUT *pg = static_cast<OUT *>(a.pgram);
-        Peaks L;
-        peaks_init(L);
+        double P[16];
+        uint32_t kf[16], flags = 0u;
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            const uint32_t row = 8u * (uint32_t)(v / 4) + 4u * kh + (uint32_t)(v % 4);
-            const uint32_t kf = kbase + 32u * row + rc;  // ascending in v
+            kf[v] = mfma_bin(kbase, rc, kh, v);
             double z[6];
+            bool ill;
             sums(v, z);
-            const double C = z[0] * iW, S = z[1] * iW;
-            const double YC = z[2] * iW - Yb * C, YS = z[3] * iW - Yb * S;
-            const double CC = 0.5 * (Wf + z[4]) * iW - C * C;
-            const double SS = 0.5 * (Wf - z[4]) * iW - S * S;
-            const double CS = 0.5 * z[5] * iW - C * S;
-            const double D = CC * SS - CS * CS;
-            const double P = (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
-            const OUT pw = (OUT)(P * scale);
-            if (kf < a.nf) {
-                if (pg) pg[(uint64_t)obj * a.nf + kf] = pw;
-                peaks_add(L, a.top_k, (double)pw, kf);
-            }
+            P[v] = gen_power(z[0] * iW, z[1] * iW, 0.5 * (Wf + z[4]) * iW, 0.5 * (Wf - z[4]) * iW, 0.5 * z[5] * iW,
+                             z[2] * iW, z[3] * iW, Yb, YY, ill);
+            if (ill && kf[v] < a.nf) flags |= 1u << v;
         }
-        if (a.top_k) write_partial<OUT, T>(a, L, obj, chunk, s_pp, s_pi);
+        recompute_flagged<true, OUT, 16>(
+            a, obj, flags, [&](uint32_t src, int v) { return mfma_bin(kbase, src & 31u, src >> 5, v); }, P);
+        mfma_store<OUT>(a, obj, chunk, kf, P, a.obj_scale[obj], s_pp, s_pi);
     }
 }
 
@@ -1031,13 +1346,13 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
 const TileEntry kSplitTiles[] = {
     // split32 on the matrix cores (standard variant): 64 threads, 32 x 32 bins
     {LSAMD_FORM_SPLIT32, 0, LSAMD_F32, 64, 16, k_sweep_mfma<float>,
-     k_combine_slices<true, false, float, 64, 16>, k_colfac},
+     k_combine_slices<true, false, float, 64, 16, true>, k_colfac<false>},
     {LSAMD_FORM_SPLIT32, 0, LSAMD_F64, 64, 16, k_sweep_mfma<double>,
-     k_combine_slices<true, false, double, 64, 16>, k_colfac},
+     k_combine_slices<true, false, double, 64, 16, true>, k_colfac<false>},
     {LSAMD_FORM_SPLIT32, 1, LSAMD_F32, 64, 16, k_sweep_mfma_gen<float>,
-     k_combine_slices<true, true, float, 64, 16>, k_colfac},
+     k_combine_slices<true, true, float, 64, 16, true>, k_colfac<true>},
     {LSAMD_FORM_SPLIT32, 1, LSAMD_F64, 64, 16, k_sweep_mfma_gen<double>,
-     k_combine_slices<true, true, double, 64, 16>, k_colfac},
+     k_combine_slices<true, true, double, 64, 16, true>, k_colfac<true>},
     // split form (FP64 rotation and sums), both output types
     LS_SPLIT(0, float, LSAMD_F32, 256, 8), LS_SPLIT(0, double, LSAMD_F64, 256, 8),
     LS_SPLIT(0, float, LSAMD_F32, 256, 4), LS_SPLIT(0, double, LSAMD_F64, 256, 4),

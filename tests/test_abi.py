@@ -165,8 +165,16 @@ def test_matrix_core_tile_workspace():
         L.lsamd_set_tile(a.form, a.variant, 0, 0)
     colfac = 12 * 64 * 16 * (163652 // 16 + 1000 + 1)
     assert mfma - valu >= colfac   # (its 1024-bin chunks also need more peak partials)
-    assert L.lsamd_set_tile(a.form, _native.GENERALIZED, 64, 16) == _native.OK  # both variants
-    assert L.lsamd_set_tile(a.form, _native.GENERALIZED, 0, 0) == _native.OK
+    # the generalized tile keeps 18 fragments per block (the weights ride on the columns)
+    a.variant = _native.GENERALIZED
+    try:
+        gen = L.lsamd_sweep_workspace_size(byref(a))
+        assert L.lsamd_set_tile(a.form, a.variant, 256, 4) == _native.OK
+        gen_valu = L.lsamd_sweep_workspace_size(byref(a))
+    finally:
+        L.lsamd_set_tile(a.form, a.variant, 0, 0)
+    assert gen - gen_valu >= colfac * 18 // 12
+    assert gen - mfma >= colfac // 2
     # the slot rule on random object sizes (including empty and 1-observation objects)
     rng = np.random.default_rng(5)
     for _ in range(20):

@@ -6,22 +6,33 @@
   bins with P >= 1e-6 P_max; the split forms at 1e-5 on P >= 1e-3 P_max.
 * The 1000-object x 1e6-bin standard batch and the generalized batch (dmag
   per object, and per observation): the plain FP64 GPU form is pinned to
-  the oracle on the first objects at the full 1e6 bins (1e-12); then the
-  split32 form (the benchmark's headline form) against that FP64 form over
-  all 1e9 bins: relative 1e-5 on bins with P >= 1e-3 P_max, the same best
-  bin for every object.
+  the oracle on the first objects at the full 1e6 bins -- standard 1e-12;
+  generalized 1e-9 on bins with P >= 1e-3 P_max, because the reference's
+  tau-frame formula over a 2000-day base is itself only that exact (its
+  second pass rounds omega (t - tau), and one ulp of tau moves weak bins by
+  >1e-11, tests/test_hiprec.py), and the GPU's error against the
+  extended-precision power (tests/hiprec.py) on the worst and on random
+  bins is at most twice the oracle's.  Then the split32 form (the
+  benchmark's headline form) against that FP64 form over all 1e9 bins:
+  relative 1e-5 on bins with P >= 1e-3 P_max, the same best bin for every
+  object.
 * The uneven batch (log-normal N with median 60, sigma 1, clipped to
   5..5000, which puts ~14% of the objects at N <= 20): its persistent,
   graph-replayed, top-3, no-periodogram split32 path against oracle.top3 of
   the FP64 oracle periodogram (the reference's three-scan rule,
-  main.cu:1165-1219): every peak power within 1e-5 relative (3e-5 for the
-  near-singular objects of N <= 8) of the oracle's power at the same bin,
-  and the same bins wherever the oracle's ranked powers are separated by
-  more than twice that tolerance.
+  main.cu:1165-1219): every peak power within 1e-5 relative of the
+  oracle's power at the same bin (1e-9 for objects of N <= 16, which the
+  sweep computes in FP64), and the same bins wherever the oracle's ranked
+  powers are separated by more than twice that tolerance.  Objects of
+  N <= 16, which the sweep computes in FP64, within 1e-7 (their near-
+  singular 2x2 systems scatter FP64 results by ~1e-9..1e-8, the oracle's
+  included; tests/test_gpu_parity.py measures both against the
+  extended-precision power).
 """
 import numpy as np
 import pytest
 
+import hiprec
 import oracle
 from gpu_lomb_scargle_amd import _native, synth
 
@@ -82,10 +93,24 @@ def _batch_full_size(torch, data, variant, oracle_objects):
                        synth.OMEGA_MAX, nf, variant == _native.GENERALIZED, F64, nthreads=16)[0]
     got = ref["pgram"][:oracle_objects].cpu().numpy()
     pm = opg.max(axis=1, keepdims=True)
-    m = opg >= 1e-6 * pm
-    rel = np.abs(got - opg)[m] / opg[m]
-    print(f"plain FP64 vs oracle ({oracle_objects} objects x {nf} bins): max rel {rel.max():.2e}")
-    assert rel.max() <= 1e-12 and (np.abs(got - opg) / pm).max() <= 1e-12
+    rel = {}
+    for frac in (1e-6, 1e-4, 1e-3):
+        m = opg >= frac * pm
+        rel[frac] = float((np.abs(got - opg)[m] / opg[m]).max())
+    absn = float((np.abs(got - opg) / pm).max())
+    print(f"plain FP64 vs oracle ({oracle_objects} objects x {nf} bins): max rel " +
+          ", ".join(f"{v:.2e} (P >= {f:g} Pmax)" for f, v in rel.items()) + f"; max |dP|/Pmax {absn:.2e}")
+    if variant == _native.STANDARD:
+        assert rel[1e-3] <= 1e-12 and absn <= 1e-12
+    else:
+        assert rel[1e-3] <= 1e-9 and absn <= 1e-10
+        bounds = np.concatenate([[0], np.cumsum(data["counts"][:oracle_objects])])
+        objs = [(data["t"][a:b], data["y"][a:b], data["dy"][a:b]) for a, b in zip(bounds[:-1], bounds[1:])]
+        c = hiprec.compare(got, opg, objs, True, synth.OMEGA_MIN, synth.OMEGA_MAX, 1e-3, 384, 384)
+        print(f"vs the extended-precision power on {c['bins']} bins: GPU max {c['gpu_max']:.2e} "
+              f"(median {c['gpu_median']:.2e}), oracle max {c['ref_max']:.2e} (median {c['ref_median']:.2e})")
+        assert c["gpu_max"] <= max(2.0 * c["ref_max"], 1e-12)
+        assert c["gpu_median"] <= max(2.0 * c["ref_median"], 1e-13)
     del opg, got
     fast = device.SweepPlan(t, y, dy, off, form=_native.FORM_SPLIT32, **kw)
     fast.run()
@@ -139,14 +164,16 @@ def test_config3_uneven_top3_vs_oracle(variant):
     _, op, oi = oracle.top3(pg, synth.OMEGA_MIN, synth.OMEGA_MAX)
     oi = oi.astype(np.int64)
     n = b["counts"]
-    tol = np.where(n <= 8, 3e-5, 1e-5)[:, None]
+    tol = np.full((K, 1), 1e-5)
     # the GPU's power at each of its bins against the oracle's at that bin
     at = np.take_along_axis(pg, gi, 1)
     rel = np.abs(gp - at) / at
     worst = np.unravel_index(np.argmax(rel / tol), rel.shape)
-    print(f"peak powers: max rel {rel[n > 8].max():.2e} (N > 8), {rel[n <= 8].max():.2e} (N <= 8, "
-          f"{int((n <= 8).sum())} objects); worst object {worst[0]} N = {n[worst[0]]}")
+    small = n <= 16  # computed in FP64 throughout
+    print(f"peak powers: max rel {rel[~small].max():.2e} (N > 16), {rel[small].max():.2e} (N <= 16, "
+          f"{int(small.sum())} objects); worst object {worst[0]} N = {n[worst[0]]}")
     assert (rel <= tol).all()
+    assert rel[small].max() <= 1e-7
     # the same bins wherever the oracle's ranking is separated by > 2 tol
     taken = np.zeros(pg.shape, bool)
     np.put_along_axis(taken, oi, True, 1)

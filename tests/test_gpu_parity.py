@@ -25,6 +25,7 @@ import os
 import numpy as np
 import pytest
 
+import hiprec
 import oracle
 from conftest import load_data
 from gpu_lomb_scargle_amd import _native, synth
@@ -506,6 +507,57 @@ def test_split32_generalized_valu_and_matrix_core_tiles(golden, dtype):
     finally:
         L.lsamd_set_tile(_native.FORM_SPLIT32, _native.GENERALIZED, 0, 0)
     assert_close(out[(64, 16)], out[(256, 4)], 2e-5, 1e-3, 4e-6)
+
+
+@pytest.mark.parametrize("variant", [_native.STANDARD, _native.GENERALIZED], ids=["standard", "generalized"])
+@pytest.mark.parametrize("tile", [(64, 16), None], ids=["matrix_core", "valu"])
+def test_small_objects_and_ill_conditioned_bins(variant, tile):
+    """Objects of 5 to 40 observations, where the 2x2 (standard) or centred
+    2x2 (generalized) system is near-singular at many frequencies (condition
+    numbers up to ~1e5 for 5 points): the split32 sweep computes objects of
+    at most 16 observations, and the bins whose condition estimate exceeds
+    its bound, in FP64 from the caller's data.  Every bin with
+    P >= 1e-3 P_max within 1e-5 of the FP64 oracle.  Objects of N <= 16
+    within 1e-7 of it, where FP64 results scatter (the oracle's own error
+    reaches ~1e-9 on a 6-observation object, tests/test_hiprec.py): against
+    the extended-precision power (tests/hiprec.py) on the worst and on
+    random bins, the GPU's error is at most twice the oracle's."""
+    gen = variant == _native.GENERALIZED
+    L = _native.core()
+    rng = np.random.default_rng(71 + variant)
+    counts = np.concatenate([np.arange(5, 21), rng.integers(5, 41, 64)])
+    ts, ys, ds, ids = [], [], [], []
+    for k, n in enumerate(counts):
+        t = np.sort(rng.integers(0, 1801, n) + rng.uniform(0, 0.3, n))
+        ts.append(t)
+        ys.append(17.0 + 0.3 * np.sin(2 * np.pi * t / rng.uniform(0.1, 5.0)) + rng.normal(0, 0.05, n))
+        ds.append(np.exp(rng.uniform(np.log(0.01), np.log(0.2), n)))
+        ids.append(np.full(n, k, np.uint32))
+    o, t, y, dy = np.concatenate(ids), np.concatenate(ts), np.concatenate(ys), np.concatenate(ds)
+    if not gen:
+        y = y - 17.0
+    nf = 30011
+    ref = oracle.batch(o, t, y, dy if gen else None, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, gen, F64,
+                       nthreads=8)[0]
+    vt = (256, 4) if gen else (256, 8)
+    try:
+        assert L.lsamd_set_tile(_native.FORM_SPLIT32, variant, *(tile or vt)) == _native.OK
+        r = _native.batch_host(o, t, y, dy if gen else None, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, variant, F64,
+                               _native.FORM_SPLIT32)
+    finally:
+        L.lsamd_set_tile(_native.FORM_SPLIT32, variant, 0, 0)
+    small = counts <= 16
+    print("N <= 16", end=": ")
+    assert_close(r["pgram"][small], ref[small], 1e-7, 1e-3, 1e-8)
+    ks = np.flatnonzero(small)
+    objs = [(t[o == k], y[o == k], dy[o == k] if gen else None) for k in ks]
+    c = hiprec.compare(r["pgram"][small], ref[small], objs, gen, synth.OMEGA_MIN, synth.OMEGA_MAX, 1e-3, 256, 256)
+    print(f"N <= 16 vs the extended-precision power on {c['bins']} bins: GPU max {c['gpu_max']:.2e} "
+          f"(median {c['gpu_median']:.2e}), oracle max {c['ref_max']:.2e} (median {c['ref_median']:.2e})")
+    assert c["gpu_max"] <= max(2.0 * c["ref_max"], 1e-12)
+    print("N 17..40", end=": ")
+    assert_close(r["pgram"][~small], ref[~small], 1e-5, 1e-3, 2e-6)
+    assert argmax_agrees(r["pgram"], ref, 1e-5) == 0
 
 
 @pytest.mark.parametrize("scale", [1e-6, 1.0, 3e5])
