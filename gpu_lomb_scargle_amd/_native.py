@@ -38,7 +38,7 @@ EXPORTED = [
     "lsamd_last_sweep_kernel_ms", "lsamd_set_tile", "lsamd_get_tile", "lsamd_version",
     "lsamd_graph_create", "lsamd_graph_launch", "lsamd_graph_destroy", "lsamd_autotune",
     "lsamd_autotune_workspace_size", "lsamd_power_grad", "lsamd_refine_peaks", "lsamd_read_table",
-    "lsamd_write_periods", "lsamd_segment_workspace_size", "lsamd_segment",
+    "lsamd_write_periods", "lsamd_segment_workspace_size", "lsamd_segment", "lsamd_sweep_static",
 ]
 
 
@@ -81,6 +81,8 @@ def _declare(L):
     L.lsamd_sweep_workspace_size.restype = c_size_t
     L.lsamd_sweep.argtypes = [POINTER(SweepArgs), c_void_p, c_size_t, c_void_p]
     L.lsamd_sweep.restype = c_int
+    L.lsamd_sweep_static.argtypes = [POINTER(SweepArgs), c_void_p]
+    L.lsamd_sweep_static.restype = c_int
     L.lsamd_batch_host.argtypes = [c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_uint64, c_double, c_double, c_uint32, c_void_p,
                                    c_uint32, c_void_p, c_void_p, c_void_p, POINTER(c_uint32)]

@@ -459,6 +459,19 @@ int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
 
 using namespace lsamd;
 
+// Workspace of lsamd_sweep_static: a device array of LSAMD_STATIC_SCRATCH_BYTES
+// on every GPU the library runs on (zero-filled at load, held until the
+// process ends).  Host code uses only its per-device address from
+// hipGetSymbolAddress.  Every sweep writes what it reads (the persistent
+// schedule's counter is cleared by a kernel), so no call depends on the
+// previous call's contents.
+namespace {
+constexpr size_t kStaticScratchBytes = LSAMD_STATIC_SCRATCH_BYTES;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+std::atomic<void *> g_scratch_ptr[kScratchDevices];
+}  // namespace
+
 extern "C" {
 
 size_t lsamd_sweep_workspace_size(const lsamd_sweep_args *a) {
@@ -507,7 +520,11 @@ double lsamd_last_sweep_kernel_ms(void) {
             return set_error(LSAMD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_bytes, void *stream_v) {
+// The sweep into a given workspace; from_scratch: the workspace is the static
+// device array of lsamd_sweep_static (no kernel timing, which would create
+// events).
+static int sweep_into(const lsamd_sweep_args *a, void *workspace, size_t workspace_bytes, void *stream_v,
+                      bool from_scratch) {
     if (!a) return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: null args");
     if (a->variant != LSAMD_STANDARD && a->variant != LSAMD_GENERALIZED)
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: bad variant");
@@ -538,6 +555,11 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
                                                 "x" + std::to_string(a->tile_fpt) + " not compiled");
     Workspace w;
     plan_workspace(a, &w);
+    if (from_scratch && w.bytes > workspace_bytes)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep_static: these arguments need " +
+                                                std::to_string(w.bytes) + " bytes of workspace, more than the " +
+                                                std::to_string(workspace_bytes) +
+                                                "-byte static scratch; use lsamd_sweep with a workspace");
     if (!workspace || workspace_bytes < w.bytes)
         return set_error(LSAMD_ERR_INVALID, "lsamd_sweep: workspace too small (need " +
                                                 std::to_string(w.bytes) + " bytes)");
@@ -666,7 +688,7 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     LS_HIP(hipStreamIsCapturing(st, &cap));
     Timing &tm = t_timing;
-    const bool timing = tm.enabled && cap == hipStreamCaptureStatusNone;
+    const bool timing = tm.enabled && cap == hipStreamCaptureStatusNone && !from_scratch;
     if (timing) {
         LS_HIP(timing_events(tm));
         LS_HIP(hipEventRecord(tm.ev[0], st));
@@ -734,6 +756,30 @@ int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_byt
         LS_HIP(hipGetLastError());
     }
     return LSAMD_OK;
+}
+
+int lsamd_sweep(const lsamd_sweep_args *a, void *workspace, size_t workspace_bytes, void *stream_v) {
+    return sweep_into(a, workspace, workspace_bytes, stream_v, false);
+}
+
+int lsamd_sweep_static(const lsamd_sweep_args *a, void *stream_v) {
+    int dev = 0;
+    LS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices)
+        return set_error(LSAMD_ERR_INVALID, "lsamd_sweep_static: device " + std::to_string(dev) +
+                                                " beyond the " + std::to_string(kScratchDevices) +
+                                                " devices it keeps a scratch address for");
+    void *ptr = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!ptr) {
+        // each device holds its own copy of the array: its address on the
+        // current device (the host-side name is not a device address)
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(LSAMD_ERR_HIP, std::string("lsamd_sweep_static: hipGetSymbolAddress(scratch): ") +
+                                                hipGetErrorString(e));
+        g_scratch_ptr[dev].store(ptr, std::memory_order_release);
+    }
+    return sweep_into(a, ptr, kStaticScratchBytes, stream_v, true);
 }
 
 struct lsamd_graph {

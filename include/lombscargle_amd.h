@@ -156,6 +156,20 @@ size_t lsamd_sweep_workspace_size(const lsamd_sweep_args *args);
 int lsamd_sweep(const lsamd_sweep_args *args, void *workspace, size_t workspace_bytes,
                 void *stream);
 
+/* lsamd_sweep without allocating and without a workspace argument: the
+ * workspace is a static device array of LSAMD_STATIC_SCRATCH_BYTES that the
+ * library holds on every GPU it runs on until the process ends.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Arguments as for lsamd_sweep; sweeps whose
+ * lsamd_sweep_workspace_size() exceeds the array fail with
+ * LSAMD_ERR_INVALID before anything is enqueued.  Every call on one GPU uses
+ * that GPU's one array: calls that overlap on one GPU (two streams, two host
+ * threads) race and must be serialised by the caller.  Kernel timing
+ * (lsamd_set_timing) is not recorded for these calls. */
+#define LSAMD_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+int lsamd_sweep_static(const lsamd_sweep_args *args, void *stream);
+
 /* Capture one sweep (all its kernels, on a non-default stream) into a HIP
  * graph for repeated launches with the same arguments and workspace; the
  * capture clears its counters with kernels, not memset nodes. */
