@@ -39,9 +39,10 @@ struct SweepArgs {
     double *part_pow;
     uint32_t *part_idx;
     int top_k;                 // peaks kept per chunk: 0, 1 or 3
-    // matrix-core tile: per-object column-factor fragments and y scale
+    // matrix-core tile: per-object column- and row-factor fragments, y scale
     uint4 *colfac;
-    float *obj_ys;
+    uint4 *rowfac;
+    float *obj_ys;  // per object: y scale, weight scale
 };
 
 typedef void (*SweepFn)(SweepArgs);
@@ -74,8 +75,14 @@ constexpr uint64_t kSliceTargetItems = 2048;
 // Matrix-core tile: column-factor fragments per 16 observations (standard:
 // y v and w; generalized: q^2 v, q (q y) v and q^2 w; head, tail, negated).
 constexpr int kColfacFragsStd = 12, kColfacFragsGen = 18;
+// Matrix-core tile: row-step phasors e^{2 pi i 32 a df t} per 16 observations
+// (lane a + 32 h: observations 8 h .. 8 h + 7 as 4 x two complex FP32).
+constexpr int kRowfacFrags = 4;
 constexpr int kPeaks = 3;      // peaks tracked per object (top-1 and top-3)
-constexpr int kGen = 8;        // doubles per object of split-generalized constants
+// doubles per object of split-generalized constants: [0..2] Wf, Yb, YY over
+// the FP32-rounded weights the VALU split sweeps apply; [3] ybar, [4] W (sum
+// of 1/dy^2); [5..7] Wf, Yb, YY over the FP64 weights w / W (matrix-core tile)
+constexpr int kGen = 8;
 constexpr double kInv2Pi = 0.159154943091895335768883763372514362;  // 1/(2 pi)
 constexpr double kTwoPi = 6.283185307179586476925286766559005768;   // 2 pi
 
@@ -122,6 +129,19 @@ __device__ __forceinline__ void peaks_insert(Peaks &L, double p, uint32_t i) {
 // nothing for top_k = 0.  Each thread adds its bins in ascending order, so an
 // equal power never displaces an earlier bin and a strict comparison of the
 // powers decides.
+// Out-of-order insertion (bins that are not the lane's next in index order):
+// the index breaks ties, as the in-order strict comparison of peaks_add does.
+__device__ __forceinline__ void peaks_merge(Peaks &L, int top_k, double p, uint32_t i) {
+    if (top_k == 1) {
+        if (better(p, i, L.p[0], L.i[0])) {
+            L.p[0] = p;
+            L.i[0] = i;
+        }
+    } else if (top_k) {
+        peaks_insert(L, p, i);
+    }
+}
+
 __device__ __forceinline__ void peaks_add(Peaks &L, int top_k, double p, uint32_t i) {
     if (top_k == 1) {
         if (p > L.p[0]) {

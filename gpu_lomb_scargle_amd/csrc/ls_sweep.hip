@@ -127,7 +127,7 @@ __global__ void k_obj_stats(StatsArgs a) {
             WY += wi * (double)y[i];
         }
         const double ybar = WY / W;
-        double Wf = 0.0, Ys = 0.0, YYs = 0.0;
+        double Wf = 0.0, Ys = 0.0, YYs = 0.0, Wd = 0.0, Yd = 0.0, YYd = 0.0;
         for (uint64_t i = b; i < e; ++i) {
             const double qi = sqrt(1.0 / ((double)dy[i] * (double)dy[i]) / W);
             const double yc = (double)y[i] - ybar;
@@ -136,6 +136,10 @@ __global__ void k_obj_stats(StatsArgs a) {
             Wf += q * q;
             Ys += (double)bq * q;
             YYs += q * q * yc * yc;
+            const double wn = 1.0 / ((double)dy[i] * (double)dy[i]) / W;
+            Wd += wn;
+            Yd += wn * yc;
+            YYd += wn * yc * yc;
         }
         const double Yb = Ys / Wf;
         a.obj_gen[kGen * k + 0] = Wf;
@@ -143,6 +147,9 @@ __global__ void k_obj_stats(StatsArgs a) {
         a.obj_gen[kGen * k + 2] = YYs / Wf - Yb * Yb;
         a.obj_gen[kGen * k + 3] = ybar;
         a.obj_gen[kGen * k + 4] = W;
+        a.obj_gen[kGen * k + 5] = Wd;
+        a.obj_gen[kGen * k + 6] = Yd / Wd;
+        a.obj_gen[kGen * k + 7] = YYd / Wd - (Yd / Wd) * (Yd / Wd);
     }
 }
 
@@ -184,7 +191,7 @@ __global__ __launch_bounds__(64) void k_obj_stats_wave(StatsArgs a) {
         }
         W = wave_sum64(W);
         const double ybar = wave_sum64(WY) / W;
-        double Wf = 0.0, Ys = 0.0, YYs = 0.0;
+        double Wf = 0.0, Ys = 0.0, YYs = 0.0, Wd = 0.0, Yd = 0.0, YYd = 0.0;
         for (uint64_t i = b + lane; i < e; i += 64) {
             const double qi = sqrt(1.0 / ((double)dy[i] * (double)dy[i]) / W);
             const double yc = (double)y[i] - ybar;
@@ -193,10 +200,17 @@ __global__ __launch_bounds__(64) void k_obj_stats_wave(StatsArgs a) {
             Wf += q * q;
             Ys += (double)bq * q;
             YYs += q * q * yc * yc;
+            const double wn = 1.0 / ((double)dy[i] * (double)dy[i]) / W;
+            Wd += wn;
+            Yd += wn * yc;
+            YYd += wn * yc * yc;
         }
         Wf = wave_sum64(Wf);
         Ys = wave_sum64(Ys);
         YYs = wave_sum64(YYs);
+        Wd = wave_sum64(Wd);
+        Yd = wave_sum64(Yd);
+        YYd = wave_sum64(YYd);
         if (lane == 0) {
             const double Yb = Ys / Wf;
             a.obj_gen[kGen * k + 0] = Wf;
@@ -204,6 +218,9 @@ __global__ __launch_bounds__(64) void k_obj_stats_wave(StatsArgs a) {
             a.obj_gen[kGen * k + 2] = YYs / Wf - Yb * Yb;
             a.obj_gen[kGen * k + 3] = ybar;
             a.obj_gen[kGen * k + 4] = W;
+            a.obj_gen[kGen * k + 5] = Wd;
+            a.obj_gen[kGen * k + 6] = Yd / Wd;
+            a.obj_gen[kGen * k + 7] = YYd / Wd - (Yd / Wd) * (Yd / Wd);
         }
     }
 }
@@ -425,9 +442,10 @@ int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
     }
     const TileEntry *te = find_tile(a->form, a->variant, a->dtype, tile.threads, tile.fpt);
     if (te && te->prep) {  // matrix-core tile: y scales and column-factor fragments
-        w->obj_ys = take(4 * K);
+        w->obj_ys = take(8 * K);
         const size_t frags = a->variant == LSAMD_GENERALIZED ? kColfacFragsGen : kColfacFragsStd;
         w->colfac = take((size_t)16 * 64 * frags * (n / 16 + K + 1));
+        w->rowfac = take((size_t)16 * 64 * kRowfacFrags * (n / 16 + K + 1));
     }
     const size_t parts = (size_t)K * w->chunks * kPeaks;
     w->part_pow = take(8 * parts);
@@ -659,6 +677,7 @@ static int sweep_into(const lsamd_sweep_args *a, void *workspace, size_t workspa
     sw.part_idx = reinterpret_cast<uint32_t *>(ws + w.part_idx);
     sw.top_k = (int)a->top_k;
     sw.colfac = w.colfac ? reinterpret_cast<uint4 *>(ws + w.colfac) : nullptr;
+    sw.rowfac = w.rowfac ? reinterpret_cast<uint4 *>(ws + w.rowfac) : nullptr;
     sw.obj_ys = w.obj_ys ? reinterpret_cast<float *>(ws + w.obj_ys) : nullptr;
 
     const uint64_t items = (uint64_t)K * w.chunks * w.slices;

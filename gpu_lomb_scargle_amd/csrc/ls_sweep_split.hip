@@ -70,12 +70,23 @@ __device__ __forceinline__ void phasor64(double f, double t, double &c, double &
     sincos_rev64(fma(f, t, -n), s, c);
 }
 
+// n / d for d > 0 (normal): the hardware reciprocal refined by two Newton
+// steps, then one correction of the quotient (within an ulp or two; the
+// IEEE division sequence is twice as long).
+__device__ __forceinline__ double fdiv(double n, double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(fma(-d, r, 1.0), r, r);
+    r = fma(fma(-d, r, 1.0), r, r);
+    const double q = n * r;
+    return fma(fma(-d, q, n), r, q);
+}
+
 // Power of the standard variant from its uncentred sums (SS = N - CC);
 // `ill` when its condition number exceeds kCondMax (or D <= 0, or NaN).
 __device__ __forceinline__ double std_power(double XC, double XS, double CC, double CS, double N, bool &ill) {
     const double SS = N - CC, D = CC * SS - CS * CS;
     ill = !((CC + SS) * (CC + SS) <= 4.0 * kCondMax * D) || !(D > 0.0);
-    return 0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS) / D;
+    return fdiv(0.5 * (SS * XC * XC - 2.0 * CS * XC * XS + CC * XS * XS), D);
 }
 
 // Power of the generalized variant from sums over weights normalised to
@@ -87,7 +98,7 @@ __device__ __forceinline__ double gen_power(double C, double S, double CCw, doub
     const double CC = CCw - C * C, SS = SSw - S * S, CS = CSw - C * S;
     const double D = CC * SS - CS * CS;
     ill = !((1.0 + C * C + S * S) * (CC + SS) <= kCondMax * D) || !(D > 0.0);
-    return (SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS) / (D * YY);
+    return fdiv(SS * YC * YC - 2.0 * CS * YC * YS + CC * YS * YS, D * YY);
 }
 
 // FP64 power of bin kf of object obj (before the object's scale); every lane
@@ -281,10 +292,10 @@ __device__ __forceinline__ void split_epilogue(const SweepArgs &a, const double 
     const double scale = a.obj_scale[obj];
     const double nobs = (double)(a.offsets[obj + 1] - a.offsets[obj]);
     double Wf = 1.0, Yb = 0.0, YY = 1.0;
-    if constexpr (GEN) {
-        Wf = a.obj_gen[kGen * obj + 0];
-        Yb = a.obj_gen[kGen * obj + 1];
-        YY = a.obj_gen[kGen * obj + 2];
+    if constexpr (GEN) {  // the matrix-core tile's column factors carry the FP64 weights
+        Wf = a.obj_gen[kGen * obj + (MFMA ? 5 : 0)];
+        Yb = a.obj_gen[kGen * obj + (MFMA ? 6 : 1)];
+        YY = a.obj_gen[kGen * obj + (MFMA ? 7 : 2)];
     }
     const double iW = 1.0 / Wf;  // one division per work item, not seven per bin
     const bool small = nobs <= (double)kSmallObs;
@@ -743,26 +754,27 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
 //   Z2[a][b] = sum_j e^{4 pi i f_a t_j} e^{4 pi i b df t_j}          = C2 + i S2
 // with CC = (N + C2) / 2, CS = S2 / 2, SS = N - CC.  Each complex sum is four
 // real products over blocks of 16 observations, formed by
-// v_mfma_f32_32x32x16_f16 with FP32 accumulation.  Every factor x, scaled by
-// 2^10, is split into an FP16 head h = fp16(x) and a tail l = fp16(x - h);
-// head x head and the two cross products carry ~22 bits (the dropped tail x
-// tail term is 2^-22 of the product); each complex sum's head x head and
-// cross products accumulate in FP32 (apart, or in one sum for Z2 of the
-// standard variant), unscaled by 2^-20 in FP64.  The head in the MFMA operand
-// must be the very value the tail was taken from (see split16).  An
-// FP16-subnormal head or tail (|x| <
-// 2^-14 after the scale, i.e. below 2^-24 of the unscaled factor) costs at
-// most that much absolute, whichever way the matrix core treats it.  Phases
-// are reduced in FP64 and evaluated by the hardware sin/cos, as in the VALU
-// split32 sweep.  The factors of Z2 are divided by their squared modulus
-// (hardware sin/cos leave |z|^2 - 1 ~ 1e-7, and CC = (N + C2) / 2 assumes
-// unit modulus).
+// v_mfma_f32_32x32x16_f16 with FP32 accumulation (each instruction adds the
+// exact sum of 8 products to the accumulator and rounds, twice: measured on
+// the device).  Every factor x is split into an FP16 head h = fp16(x) and a
+// tail l = fp16(x - h); head x head and the two cross products carry ~22 bits
+// (the dropped tail x tail term is 2^-22 of the product); each complex sum's
+// head x head and cross products accumulate in FP32 (apart, or in one sum for
+// Z2 of the standard variant), unscaled in FP64.  The head in the MFMA
+// operand must be the very value the tail was taken from (see split16x2).
+// The row factor of row a is u = u0 R: u0 = e^{2 pi i f0 t} with f0 the
+// item's first bin, one per observation and item (FP64 phase and sin/cos,
+// staged in LDS), and R = e^{2 pi i 32 a df t}, one per observation and row
+// of the object (k_colfac, FP64 phase and sin/cos); their FP32 product costs
+// four FP32 operations per row and observation, and |u| = 1 to a few FP32
+// ulps, which CC = (N + C2) / 2 assumes.  Row factors are unscaled (|u| <= 1:
+// an FP16-subnormal head or tail costs at most 2^-24 absolute, whichever way
+// the matrix core treats it); column factors are scaled by 2^10.
 // The column factor depends only on the object: k_colfac writes its operand
 // fragments once per 16 observations, and y is scaled per object by a power
 // of two to max |y| in [0.5, 1) (FP16 range; undone exactly in the power).
-// Observations past the object's end have a zero column factor, and the
-// staging pads the row-side records with zeros, so partial blocks need no
-// masking.
+// Observations past the object's end have zero row and column factors, so
+// partial blocks need no masking.
 // Operand layout of 32x32x16 (checked on the device): lane l holds row (A) or
 // column (B) l % 32 and observations 8 (l / 32) .. +7 of the block; the
 // result holds column l % 32, rows 8 (v / 4) + 4 (l / 32) + v % 4, v < 16.
@@ -770,7 +782,7 @@ __global__ __launch_bounds__(T) void k_sweep_split64(SweepArgs a) {
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-constexpr float kFacScale = 1024.0f;  // 2^10 on each factor
+constexpr float kFacScale = 1024.0f;  // 2^10 on the column factors
 
 // Heads and tails of two factors x: h = fp16(x), l = fp16(x - h) (x - h is
 // exact in FP32).  The head must be one value.  When x is a product a b, the compiler may
@@ -797,75 +809,130 @@ __device__ __forceinline__ void split16x2(float x0, float x1, half2_t &h, half2_
 // next object's floor((beg + N)/16) + k + 1.
 __device__ __forceinline__ uint64_t colfac_slot(uint64_t beg, uint32_t obj) { return (beg >> 4) + obj; }
 
+// Heads and tails of two FP64 factors: h = fp16(fp32(x)), l = fp16(x - h),
+// the remainder x - h exact in FP64 (~22 bits of x in h + l).
+__device__ __forceinline__ void split64x2(double x0, double x1, half2_t &h, half2_t &l) {
+    h = __builtin_convertvector((float2_t){(float)x0, (float)x1}, half2_t);
+    l = __builtin_convertvector((float2_t){(float)(x0 - (double)h.x), (float)(x1 - (double)h.y)}, half2_t);
+}
+
 // Wavefronts (object, y) with y < gridDim.y write the column-factor fragments
 // of 16-observation blocks y, y + gridDim.y, ... of the object, each factor
 // as head, tail (and negated head, tail where a product needs -f_i):
 //   standard:    y v (6 fragments: r, i, -i), w (6)
-//   generalized: q^2 v (6), q (q y_c) v (6), q^2 w (6)
-// with v = e^{2 pi i b df t} (b < 32 the column), w = v^2 / |v|^2, y the
-// object's y scaled by its power of two ys (wavefront y = 0 also stores ys),
-// q^2 the normalised weight and q y_c the records' weighted centred y.
-// Observations past the object's end have zero factors.
-template <bool GEN>
+//   generalized: (w/W) v (6), (w/W) y_c v (6), (w/W) w (6)
+// with v = e^{2 pi i b df t} (b < 32 the column) and w = v^2, from the
+// caller's FP64 (or FP32) t, y, dy in FP64 arithmetic: FP64 phase and
+// sin/cos, w/W = 1 / (dy^2 W) the normalised weight, y_c = y - ybar; y (or
+// (w/W) y_c) scaled per object by a power of two ys, and w/W by a power of
+// two ws, to a maximum in [0.5, 1), then all by 2^10 (FP16 range; the scales
+// are undone exactly in the power; wavefront y = 0 stores them); and the
+// row-step fragments (e^{2 pi i 32 a df t}, row a = l % 32).  Observations
+// past the object's end have zero factors.
+template <bool GEN, typename IN>
 __global__ __launch_bounds__(64) void k_colfac(SweepArgs a) {
     constexpr int NF = GEN ? kColfacFragsGen : kColfacFragsStd;
     const uint32_t k = blockIdx.x, l = threadIdx.x;
     const uint64_t beg = a.offsets[k], end = a.offsets[k + 1];
     const uint32_t n = (uint32_t)(end - beg);
-    float m = 0.0f;
-    for (uint64_t i = beg + l; i < end; i += 64) m = fmaxf(m, fabsf(a.rec_a[i].x));
+    const IN *y = static_cast<const IN *>(a.y);
+    const IN *dy = static_cast<const IN *>(a.dy);
+    double iW = 1.0, ybar = 0.0;
+    if constexpr (GEN) {
+        iW = 1.0 / a.obj_gen[kGen * k + 4];
+        ybar = a.obj_gen[kGen * k + 3];
+    }
+    auto weight = [&](uint64_t i) -> double {  // w / W (1 for the standard variant)
+        if constexpr (GEN) {
+            const double d = (double)dy[i];
+            return iW / (d * d);
+        } else {
+            return 1.0;
+        }
+    };
+    auto yfac = [&](uint64_t i) -> double {  // y, or (w / W) (y - ybar)
+        if constexpr (GEN) return weight(i) * ((double)y[i] - ybar);
+        else return (double)y[i];
+    };
+    double my = 0.0, mw = 0.0;
+    for (uint64_t i = beg + l; i < end; i += 64) {
+        my = fmax(my, fabs(yfac(i)));
+        mw = fmax(mw, weight(i));
+    }
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-    int ex = 0;
-    (void)frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
-    const float ys = m > 0.0f ? ldexpf(1.0f, -ex) : 1.0f;
-    if (blockIdx.y == 0 && l == 0) a.obj_ys[k] = ys;
+    for (int off = 32; off >= 1; off >>= 1) {
+        my = fmax(my, __shfl_xor(my, off, 64));
+        mw = fmax(mw, __shfl_xor(mw, off, 64));
+    }
+    int ey = 0, ew = 0;
+    (void)frexp(my, &ey);  // my = f 2^ey, f in [0.5, 1)
+    (void)frexp(mw, &ew);
+    const double ys = my > 0.0 ? ldexp(1.0, -ey) : 1.0;
+    const double ws = mw > 0.0 ? ldexp(1.0, -ew) : 1.0;
+    if (blockIdx.y == 0 && l == 0) {
+        a.obj_ys[2 * k] = (float)ys;
+        a.obj_ys[2 * k + 1] = (float)ws;
+    }
+    const double sy = ys * (double)kFacScale, sw = ws * (double)kFacScale;  // exact: powers of two
     const double fb = (double)(l & 31u) * a.df_rev;
+    const double fr = (double)(32u * (l & 31u)) * a.df_rev;  // row step of row l % 32
     uint4 *dst = a.colfac + colfac_slot(beg, k) * (uint64_t)(NF * 64) + l;
+    uint4 *rdst = a.rowfac + colfac_slot(beg, k) * (uint64_t)(kRowfacFrags * 64) + l;
     for (uint32_t q = blockIdx.y; q * 16u < n; q += gridDim.y) {
+        // row-step phasors R = e^{2 pi i 32 (l % 32) df t} of observations
+        // 8 (l / 32) .. + 7, in FP64 rounded to FP32 (zero past the end)
+#pragma unroll
+        for (int p = 0; p < kRowfacFrags; ++p) {
+            float r4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const uint32_t j = q * 16u + 8u * (l >> 5) + (uint32_t)(2 * p + e);
+                if (j < n) {
+                    double c, sn;
+                    phasor64(fr, a.rec_t[beg + j], c, sn);
+                    r4[2 * e] = (float)c;
+                    r4[2 * e + 1] = (float)sn;
+                }
+            }
+            uint4 u;
+            __builtin_memcpy(&u, r4, sizeof(u));
+            rdst[(uint64_t)(q * kRowfacFrags + p) * 64] = u;
+        }
         half8_t f[NF];
 #pragma unroll
         for (int v = 0; v < 8; v += 2) {
-            float x[NF / 3][2];  // the factors of observations v, v + 1: real, imaginary per sum
+            double x[NF / 3][2];  // the factors of observations v, v + 1: real, imaginary per sum
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const uint32_t j = q * 16u + 8u * (l >> 5) + (uint32_t)(v + e);
-                float cb = 0.0f, sb = 0.0f, yv = 0.0f, qv = 0.0f;
+                double cb = 0.0, sb = 0.0, yv = 0.0, wv = 0.0;
                 if (j < n) {
-                    double pb = fb * a.rec_t[beg + j];
-                    pb -= __builtin_rint(pb);
-                    cb = __builtin_amdgcn_cosf((float)pb);
-                    sb = __builtin_amdgcn_sinf((float)pb);
-                    const float4 r = a.rec_a[beg + j];
-                    yv = r.x * ys;  // exact: a power of two
-                    qv = r.y;
+                    phasor64(fb, a.rec_t[beg + j], cb, sb);
+                    yv = yfac(beg + j) * sy;
+                    wv = weight(beg + j) * sw;
                 }
-                // w = v^2 / |v|^2 (1/n ~ 2 - n for n ~ 1), scaled
-                const float ssb = sb * sb, gb = kFacScale * (2.0f - fmaf(cb, cb, ssb));
-                const float wr = fmaf(cb, cb, -ssb) * gb, wi = 2.0f * cb * sb * gb;
-                const float c1 = cb * kFacScale, s1 = sb * kFacScale;  // exact: a power of two
+                const double wr = cb * cb - sb * sb, wi = 2.0 * cb * sb;  // v^2
                 if constexpr (GEN) {
-                    const float q2 = qv * qv, qy = qv * yv;
-                    x[0][e] = q2 * c1;
-                    x[1][e] = q2 * s1;
-                    x[2][e] = qy * c1;
-                    x[3][e] = qy * s1;
-                    x[4][e] = q2 * wr;
-                    x[5][e] = q2 * wi;
+                    x[0][e] = wv * cb;
+                    x[1][e] = wv * sb;
+                    x[2][e] = yv * cb;
+                    x[3][e] = yv * sb;
+                    x[4][e] = wv * wr;
+                    x[5][e] = wv * wi;
                 } else {
-                    x[0][e] = yv * c1;
-                    x[1][e] = yv * s1;
-                    x[2][e] = wr;
-                    x[3][e] = wi;
+                    x[0][e] = yv * cb;
+                    x[1][e] = yv * sb;
+                    x[2][e] = wv * wr;
+                    x[3][e] = wv * wi;
                 }
             }
 #pragma unroll
             for (int p = 0; p < NF / 6; ++p) {  // each complex factor: r (h, l), i (h, l), -i (h, l)
                 half2_t h, t;
-                split16x2(x[2 * p][0], x[2 * p][1], h, t);
+                split64x2(x[2 * p][0], x[2 * p][1], h, t);
                 f[6 * p + 0][v] = h.x; f[6 * p + 0][v + 1] = h.y;
                 f[6 * p + 1][v] = t.x; f[6 * p + 1][v + 1] = t.y;
-                split16x2(x[2 * p + 1][0], x[2 * p + 1][1], h, t);
+                split64x2(x[2 * p + 1][0], x[2 * p + 1][1], h, t);
                 f[6 * p + 2][v] = h.x; f[6 * p + 2][v + 1] = h.y;
                 f[6 * p + 3][v] = t.x; f[6 * p + 3][v + 1] = t.y;
                 f[6 * p + 4][v] = -h.x; f[6 * p + 4][v + 1] = -h.y;
@@ -881,27 +948,32 @@ __global__ __launch_bounds__(64) void k_colfac(SweepArgs a) {
     }
 }
 
-// Row factors of a 16-observation block (t in LDS from s_t[j0]) for lane
-// (row rc, half kh): u = e^{2 pi i fa t} and u^2 / |u|^2 of the lane's 8
-// observations, scaled by 2^10, as FP16 heads and tails.
+// Row factors of a 16-observation block for lane (row rc, half kh): u = u0 R
+// with u0 = e^{2 pi i f0 t} (f0 the item's first bin; LDS, from s_u0[j0])
+// and R = e^{2 pi i 32 rc df t} (the row-step fragments rr), and u^2, as FP16
+// heads and tails.  Both phasors come from FP64 phases and FP64 sin/cos
+// rounded to FP32, so |u| = 1 to a few FP32 ulps and u^2 needs no
+// normalisation; the factors are unscaled (|u| <= 1).
 struct RowFrags {
     half8_t ur_h, ur_l, ui_h, ui_l, qr_h, qr_l, qi_h, qi_l;
 };
-__device__ __forceinline__ void row_factors(const double *s_t, uint32_t j0, uint32_t kh, double fa, RowFrags &R) {
+__device__ __forceinline__ void row_factors(const float2 *s_u0, uint32_t j0, uint32_t kh,
+                                            const uint4 (&rr)[kRowfacFrags], RowFrags &R) {
 #pragma unroll
     for (int v = 0; v < 8; v += 2) {
-        float x[4][2];  // u_r, u_i, (u^2/|u|^2)_r, (u^2/|u|^2)_i
+        float x[4][2];  // u_r, u_i, (u^2)_r, (u^2)_i
+        const uint4 r = rr[v >> 1];
+        const float rx[2] = {__uint_as_float(r.x), __uint_as_float(r.z)};
+        const float ry[2] = {__uint_as_float(r.y), __uint_as_float(r.w)};
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            double ph = fa * s_t[j0 + 8u * kh + (uint32_t)(v + e)];
-            ph -= __builtin_rint(ph);
-            const float c = __builtin_amdgcn_cosf((float)ph);
-            const float s = __builtin_amdgcn_sinf((float)ph);
-            x[0][e] = c * kFacScale;  // exact: a power of two
-            x[1][e] = s * kFacScale;
-            const float ss = s * s, g = kFacScale * (2.0f - fmaf(c, c, ss));
-            x[2][e] = fmaf(c, c, -ss) * g;
-            x[3][e] = (c + c) * s * g;
+            const float2 b = s_u0[j0 + 8u * kh + (uint32_t)(v + e)];  // < T: j0 <= T - 16
+            const float ur = fmaf(b.x, rx[e], -(b.y * ry[e]));
+            const float ui = fmaf(b.x, ry[e], b.y * rx[e]);
+            x[0][e] = ur;
+            x[1][e] = ui;
+            x[2][e] = fmaf(ur, ur, -(ui * ui));
+            x[3][e] = (ur + ur) * ui;
         }
         half2_t h, t;
         split16x2(x[0][0], x[0][1], h, t);
@@ -919,6 +991,25 @@ __device__ __forceinline__ void row_factors(const double *s_t, uint32_t j0, uint
     }
 }
 
+// Loads the row-step fragments of one 16-observation block.
+__device__ __forceinline__ void load_rows(const uint4 *rf, uint4 (&rr)[kRowfacFrags]) {
+#pragma unroll
+    for (int i = 0; i < kRowfacFrags; ++i) rr[i] = rf[i * 64];
+}
+
+// e^{2 pi i f0 t} of observation b0 + l of the current stage into s_u0[l]
+// (zero past cnt, the object's end: zero row factors).
+__device__ __forceinline__ void stage_u0(const SweepArgs &a, uint64_t b0, uint32_t cnt, double f0, float2 *s_u0) {
+    const uint32_t l = threadIdx.x;
+    float2 u0 = make_float2(0.0f, 0.0f);
+    if (l < cnt) {
+        double c, sn;
+        phasor64(f0, a.rec_t[b0 + l], c, sn);
+        u0 = make_float2((float)c, (float)sn);
+    }
+    s_u0[l] = u0;
+}
+
 // Loads the column-factor fragments of one 16-observation block.
 template <int NF>
 __device__ __forceinline__ void load_cols(const uint4 *cf, half8_t (&col)[NF]) {
@@ -930,6 +1021,55 @@ __device__ __forceinline__ void load_cols(const uint4 *cf, half8_t (&col)[NF]) {
 }
 
 #define LS_MFMA(ACC, A, B) ACC = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B, ACC, 0, 0, 0)
+
+// Emits a lane's 16 bins of a matrix-core work item.  power(v, ill) gives
+// result entry v's power (before the object's scale) and whether its
+// condition estimate exceeds the bound; the other bins are stored and enter
+// the peaks at once (kf ascends in v), the flagged ones (rare) are computed
+// again in FP64 afterwards, one at a time with the whole wavefront, and merged
+// with the index tie-break, so that no array of 16 powers stays live.
+template <bool GEN, typename OUT, typename PowerOf>
+__device__ __forceinline__ void mfma_emit(const SweepArgs &a, uint32_t obj, uint32_t chunk, uint32_t kbase,
+                                          PowerOf power, double (*s_pp)[kPeaks], uint32_t (*s_pi)[kPeaks]) {
+    const uint32_t l = threadIdx.x, rc = l & 31u, kh = l >> 5;
+    const double scale = a.obj_scale[obj];
+    OUT *pg = static_cast<OUT *>(a.pgram);
+    const uint64_t row = (uint64_t)obj * a.nf;
+    Peaks L;
+    peaks_init(L);
+    uint32_t flags = 0u;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const uint32_t kf = mfma_bin(kbase, rc, kh, v);
+        bool ill;
+        const double P = power(v, ill);
+        if (kf < a.nf) {
+            if (ill) {
+                flags |= 1u << v;
+            } else {
+                const OUT pw = (OUT)(P * scale);
+                if (pg) pg[row + kf] = pw;
+                peaks_add(L, a.top_k, (double)pw, kf);
+            }
+        }
+    }
+    uint64_t lanes = __ballot(flags != 0u);
+    while (lanes) {
+        const uint32_t src = (uint32_t)__builtin_ctzll(lanes);
+        const uint32_t f = (uint32_t)__shfl((int)flags, (int)src, 64);
+        const uint32_t kf = mfma_bin(kbase, src & 31u, src >> 5, __builtin_ctz(f));
+        const double pe = exact_power<GEN, OUT>(a, obj, kf);
+        if (l == src) {
+            const OUT pw = (OUT)(pe * scale);
+            if (pg) pg[row + kf] = pw;
+            peaks_merge(L, a.top_k, (double)pw, kf);
+            flags &= flags - 1u;
+        }
+        lanes = __ballot(flags != 0u);
+    }
+    if (a.top_k) write_partial<OUT, 64>(a, L, obj, chunk, s_pp, s_pi);
+}
+
 
 // Stores a lane's 16 powers (scaled by the object's scale) and merges the
 // item's peaks; kf ascends in v.
@@ -960,13 +1100,13 @@ template <typename OUT>
 __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
     constexpr int T = 64, M = 16;  // 64 threads, 1024 bins per item
     constexpr uint32_t kFoldObs = 256;  // MFMA accumulation length
-    __shared__ double s_t[T];
+    __shared__ float2 s_u0[T];
     __shared__ float s_acc[4][16][T];  // sums of the 256-observation partials XC, XS, C2, S2
     __shared__ double s_pp[1][kPeaks];
     __shared__ uint32_t s_pi[1][kPeaks];
     __shared__ uint64_t s_item;
     const uint32_t l = threadIdx.x, rc = l & 31u, kh = l >> 5;
-    constexpr double kUnscale = 1.0 / ((double)kFacScale * (double)kFacScale);
+    constexpr double kUnscale = 1.0 / (double)kFacScale;  // column factors scaled, rows not
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
          item = next_item(a, &s_item, item)) {
         // (object, chunk, slice); slices hold a multiple of 16 observations
@@ -996,9 +1136,11 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
             continue;
         }
         constexpr int NF = kColfacFragsStd;
-        const float ys = a.obj_ys[obj];
-        const double fa = a.fmin_rev + (double)(kbase + 32u * rc) * a.df_rev;  // row rc, column 0
-        const uint4 *cf = a.colfac + (colfac_slot(obeg, obj) + ((beg - obeg) >> 4)) * (uint64_t)(NF * 64) + l;
+        const float ys = a.obj_ys[2 * obj], ws = a.obj_ys[2 * obj + 1];
+        const double f0 = a.fmin_rev + (double)kbase * a.df_rev;  // the item's first bin
+        const uint64_t slot = colfac_slot(obeg, obj) + ((beg - obeg) >> 4);
+        const uint4 *cf = a.colfac + slot * (uint64_t)(NF * 64) + l;
+        const uint4 *rf = a.rowfac + slot * (uint64_t)(kRowfacFrags * 64) + l;
         f32x16_t z1r = {}, z1r_x = {}, z1i = {}, z1i_x = {};  // Z1: head x head, cross products
         f32x16_t z2r = {}, z2i = {};
         // after every kFoldObs observations the accumulators move into FP32
@@ -1023,20 +1165,26 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
             z2i = f32x16_t{};
             folded = true;
         };
+        // the row-step fragments are loaded one block ahead (their latency
+        // would otherwise stall the row factors of every block)
+        uint4 rr[kRowfacFrags];
+        load_rows(rf, rr);
         for (uint64_t g0 = beg; g0 < end; g0 += kFoldObs) {
         const uint64_t gend = (end - g0) < kFoldObs ? end : g0 + kFoldObs;
         for (uint64_t b0 = g0; b0 < gend; b0 += T) {
             const uint32_t cnt = (uint32_t)((gend - b0) < (uint64_t)T ? (gend - b0) : (uint64_t)T);
             __syncthreads();  // also orders the previous item's peak merge before this one's
-            // past cnt (the object's end) t = 0: finite row factors against
-            // zero column factors
-            s_t[l] = l < cnt ? a.rec_t[b0 + l] : 0.0;
+            stage_u0(a, b0, cnt, f0, s_u0);
             __syncthreads();
-            for (uint32_t j0 = 0; j0 < cnt; j0 += 16, cf += NF * 64) {
+            for (uint32_t j0 = 0; j0 < cnt; j0 += 16, cf += NF * 64, rf += kRowfacFrags * 64) {
                 half8_t col[NF];
                 load_cols<NF>(cf, col);
                 RowFrags R;
-                row_factors(s_t, j0, kh, fa, R);
+                row_factors(s_u0, j0, kh, rr, R);
+                // the next block's, loaded while this one's MFMAs run (one slot
+                // past an object's or slice's last block is still inside the
+                // table: at most slot n/16 + K of n/16 + K + 1)
+                load_rows(rf + kRowfacFrags * 64, rr);
                 // Re Z1 = u_r (y v)_r - u_i (y v)_i,  Im Z1 = u_r (y v)_i + u_i (y v)_r
                 LS_MFMA(z1r, R.ur_h, col[0]);
                 LS_MFMA(z1r_x, R.ur_h, col[1]);
@@ -1080,12 +1228,12 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
                 C2 += (double)e[2 * 16 * T];
                 S2 += (double)e[3 * 16 * T];
             }
-            XC *= kUnscale;
-            XS *= kUnscale;
-            C2 *= kUnscale;
-            S2 *= kUnscale;
+            const double uy = kUnscale / (double)ys, uw = kUnscale / (double)ws;  // powers of two
+            XC *= uy;
+            XS *= uy;
+            C2 *= uw;
+            S2 *= uw;
         };
-        const double iys = 1.0 / (double)ys;  // a power of two
         if (a.slices > 1) {
             // the slice's FP64 sums XC, XS, CC, CS of result entry v at
             // [(q M + v) T + l], the matrix-core layout k_combine_slices reads
@@ -1096,28 +1244,19 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
             for (int v = 0; v < 16; ++v) {
                 double XC, XS, C2, S2;
                 sums(v, XC, XS, C2, S2);
-                dst[(0 * M + v) * T] = XC * iys;
-                dst[(1 * M + v) * T] = XS * iys;
+                dst[(0 * M + v) * T] = XC;
+                dst[(1 * M + v) * T] = XS;
                 dst[(2 * M + v) * T] = 0.5 * (ns + C2);
                 dst[(3 * M + v) * T] = 0.5 * S2;
             }
             continue;
         }
         const double nobs = (double)(oend - obeg);
-        double P[16];
-        uint32_t kf[16], flags = 0u;
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            kf[v] = mfma_bin(kbase, rc, kh, v);
+        mfma_emit<false, OUT>(a, obj, chunk, kbase, [&](int v, bool &ill) {
             double XC, XS, C2, S2;
-            bool ill;
             sums(v, XC, XS, C2, S2);
-            P[v] = std_power(XC * iys, XS * iys, 0.5 * (nobs + C2), 0.5 * S2, nobs, ill);
-            if (ill && kf[v] < a.nf) flags |= 1u << v;
-        }
-        recompute_flagged<false, OUT, 16>(
-            a, obj, flags, [&](uint32_t src, int v) { return mfma_bin(kbase, src & 31u, src >> 5, v); }, P);
-        mfma_store<OUT>(a, obj, chunk, kf, P, a.obj_scale[obj], s_pp, s_pi);
+            return std_power(XC, XS, 0.5 * (nobs + C2), 0.5 * S2, nobs, ill);
+        }, s_pp, s_pi);
     }
 }
 
@@ -1128,26 +1267,26 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
 // give the seven weighted sums of the generalized power: C, S = Zw;
 // YC, YS = Zy; CC, SS = (Wf +- Re Z2) / 2; CS = Im Z2 / 2, with Wf = sum q^2.
 // 36 MFMAs per 16 observations.
-// Zy sets the precision of weak bins (P ~ 1e-3 P_max): every 16-observation
-// block forms it in zeroed FP32 accumulators (head x head and cross products
-// apart), so that their rounding errors scale with the block's sum rather
-// than the running one, and adds it to FP64 running sums in registers.  The
-// weight-only Zw and Z2 (head x head and cross products apart) accumulate
-// across 256 observations and fold into FP32 sums in LDS; they feed the
-// centred CC - C^2, SS - S^2, CS - C S, which cancel at aliasing frequencies,
-// and those bins exceed the condition bound and are recomputed in FP64.
-// One wavefront per SIMD (more than 256 registers per lane); 16 KiB of LDS.
+// All three sums accumulate in FP32 (head x head and cross products apart)
+// across 256 observations and then fold into sums in LDS: Zy, which sets the
+// precision of weak bins (P ~ 1e-3 P_max), into FP64; the weight-only Zw and
+// Z2 into FP32 (they feed the centred CC - C^2, SS - S^2, CS - C S, which
+// cancel at aliasing frequencies, and those bins exceed the condition bound
+// and are recomputed in FP64).  Each MFMA adds exact 8-product sums to its
+// accumulator, so restarting Zy per 16-observation block gains nothing.
+// One wavefront per SIMD (more than 256 registers per lane); 32 KiB of LDS.
 template <typename OUT>
 __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
     constexpr int T = 64, M = 16;
     constexpr uint32_t kFoldObs = 256;
-    __shared__ double s_t[T];
-    __shared__ float s_accw[4][16][T];  // FP32 Re/Im of Zw and Z2 of a lane's 16 bins
+    __shared__ float2 s_u0[T];
+    __shared__ float s_accw[4][16][T];   // FP32 Re/Im of Zw and Z2 of a lane's 16 bins
+    __shared__ double s_accy[2][16][T];  // FP64 Re/Im of Zy
     __shared__ double s_pp[1][kPeaks];
     __shared__ uint32_t s_pi[1][kPeaks];
     __shared__ uint64_t s_item;
     const uint32_t l = threadIdx.x, rc = l & 31u, kh = l >> 5;
-    constexpr double kUnscale = 1.0 / ((double)kFacScale * (double)kFacScale);
+    constexpr double kUnscale = 1.0 / (double)kFacScale;  // column factors scaled, rows not
     for (uint64_t item = first_item(a, &s_item); item < a.total_items;
          item = next_item(a, &s_item, item)) {
         uint64_t oc = item;
@@ -1176,17 +1315,16 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
             continue;
         }
         constexpr int NF = kColfacFragsGen;
-        const float ys = a.obj_ys[obj];
-        const double fa = a.fmin_rev + (double)(kbase + 32u * rc) * a.df_rev;
-        const uint4 *cf = a.colfac + (colfac_slot(obeg, obj) + ((beg - obeg) >> 4)) * (uint64_t)(NF * 64) + l;
-        f32x16_t zw[4] = {}, z2[4] = {};  // {Re hh, Re x, Im hh, Im x} since the last fold
-        double ry[2][16];                  // FP64 running Re, Im of Zy
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) ry[q][v] = 0.0;
-        // entry (q, v) of this lane's LDS sums: acc_l[(16 q + v) T]
+        const float ys = a.obj_ys[2 * obj], ws = a.obj_ys[2 * obj + 1];
+        const double f0 = a.fmin_rev + (double)kbase * a.df_rev;  // the item's first bin
+        const uint64_t slot = colfac_slot(obeg, obj) + ((beg - obeg) >> 4);
+        const uint4 *cf = a.colfac + slot * (uint64_t)(NF * 64) + l;
+        const uint4 *rf = a.rowfac + slot * (uint64_t)(kRowfacFrags * 64) + l;
+        // {Re hh, Re x, Im hh, Im x} since the last fold
+        f32x16_t zw[4] = {}, z2[4] = {}, zy[4] = {};
+        // entry (q, v) of this lane's LDS sums: acc_l[(16 q + v) T], acy_l likewise
         float *const acc_l = &s_accw[0][0][l];
+        double *const acy_l = &s_accy[0][0][l];
         bool folded = false;
         auto fold = [&]() {
 #pragma unroll
@@ -1196,28 +1334,38 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
                 e[1 * 16 * T] = (folded ? e[1 * 16 * T] : 0.0f) + (zw[2][v] + zw[3][v]);
                 e[2 * 16 * T] = (folded ? e[2 * 16 * T] : 0.0f) + (z2[0][v] + z2[1][v]);
                 e[3 * 16 * T] = (folded ? e[3 * 16 * T] : 0.0f) + (z2[2][v] + z2[3][v]);
+                double *d = acy_l + v * T;
+                d[0] = (folded ? d[0] : 0.0) + ((double)zy[0][v] + (double)zy[1][v]);
+                d[16 * T] = (folded ? d[16 * T] : 0.0) + ((double)zy[2][v] + (double)zy[3][v]);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 zw[i] = f32x16_t{};
                 z2[i] = f32x16_t{};
+                zy[i] = f32x16_t{};
             }
             folded = true;
         };
+        // the row-step fragments are loaded one block ahead (their latency
+        // would otherwise stall the row factors of every block)
+        uint4 rr[kRowfacFrags];
+        load_rows(rf, rr);
         for (uint64_t g0 = beg; g0 < end; g0 += kFoldObs) {
         const uint64_t gend = (end - g0) < kFoldObs ? end : g0 + kFoldObs;
         for (uint64_t b0 = g0; b0 < gend; b0 += T) {
             const uint32_t cnt = (uint32_t)((gend - b0) < (uint64_t)T ? (gend - b0) : (uint64_t)T);
             __syncthreads();  // also orders the previous item's peak merge before this one's
-            // past cnt (the object's end) t = 0: finite row factors against
-            // zero column factors
-            s_t[l] = l < cnt ? a.rec_t[b0 + l] : 0.0;
+            stage_u0(a, b0, cnt, f0, s_u0);
             __syncthreads();
-            for (uint32_t j0 = 0; j0 < cnt; j0 += 16, cf += NF * 64) {
+            for (uint32_t j0 = 0; j0 < cnt; j0 += 16, cf += NF * 64, rf += kRowfacFrags * 64) {
                 half8_t col[NF];
                 load_cols<NF>(cf, col);
                 RowFrags R;
-                row_factors(s_t, j0, kh, fa, R);
+                row_factors(s_u0, j0, kh, rr, R);
+                // the next block's, loaded while this one's MFMAs run (one slot
+                // past an object's or slice's last block is still inside the
+                // table: at most slot n/16 + K of n/16 + K + 1)
+                load_rows(rf + kRowfacFrags * 64, rr);
                 // Re Z = u_r f_r - u_i f_i,  Im Z = u_r f_i + u_i f_r for the
                 // column factors f = q^2 v (col 0..5) of Zw, q (q y) v (6..11)
                 // of Zy, and, with u^2/|u|^2 for u, q^2 w (12..17) of Z2;
@@ -1246,8 +1394,6 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
                 LS_MFMA(z2[2], R.qi_h, col[12]);
                 LS_MFMA(z2[3], R.qi_h, col[13]);
                 LS_MFMA(z2[3], R.qi_l, col[12]);
-                // the block's Zy from zero, cross products first
-                f32x16_t zy[4] = {};
                 LS_MFMA(zy[1], R.ur_h, col[7]);
                 LS_MFMA(zy[1], R.ur_l, col[6]);
                 LS_MFMA(zy[1], R.ui_h, col[11]);
@@ -1260,11 +1406,6 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
                 LS_MFMA(zy[3], R.ui_l, col[6]);
                 LS_MFMA(zy[2], R.ur_h, col[8]);
                 LS_MFMA(zy[2], R.ui_h, col[6]);
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    ry[0][v] += (double)(zy[0][v] + zy[1][v]);
-                    ry[1][v] += (double)(zy[2][v] + zy[3][v]);
-                }
             }
         }
         if (gend < end) fold();
@@ -1273,8 +1414,8 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
         auto sums = [&](int v, double (&z)[6]) {
             z[0] = (double)zw[0][v] + (double)zw[1][v];
             z[1] = (double)zw[2][v] + (double)zw[3][v];
-            z[2] = ry[0][v];
-            z[3] = ry[1][v];
+            z[2] = (double)zy[0][v] + (double)zy[1][v];
+            z[3] = (double)zy[2][v] + (double)zy[3][v];
             z[4] = (double)z2[0][v] + (double)z2[1][v];
             z[5] = (double)z2[2][v] + (double)z2[3][v];
             if (folded) {
@@ -1283,18 +1424,23 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
                 z[1] += (double)e[1 * 16 * T];
                 z[4] += (double)e[2 * 16 * T];
                 z[5] += (double)e[3 * 16 * T];
+                const double *d = acy_l + v * T;
+                z[2] += d[0];
+                z[3] += d[16 * T];
             }
-            const double iys = 1.0 / (double)ys;
+            const double uy = kUnscale / (double)ys, uw = kUnscale / (double)ws;  // powers of two
 #pragma unroll
-            for (int q = 0; q < 6; ++q) z[q] *= kUnscale * (q == 2 || q == 3 ? iys : 1.0);
+            for (int q = 0; q < 6; ++q) z[q] *= (q == 2 || q == 3) ? uy : uw;
         };
         if (a.slices > 1) {
             // the slice's seven FP64 sums in the matrix-core layout
             // k_combine_slices reads, with the slice's weight sum for CC and SS
-            double wsum = 0.0;
+            double wsum = 0.0;  // of w / W, as the column factors
+            const OUT *dy = static_cast<const OUT *>(a.dy);
+            const double iWt = 1.0 / a.obj_gen[kGen * obj + 4];
             for (uint64_t i = beg + l; i < end; i += T) {
-                const float q = a.rec_a[i].y;
-                wsum += (double)(q * q);
+                const double d = (double)dy[i];
+                wsum += iWt / (d * d);
             }
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) wsum += __shfl_xor(wsum, off, 64);
@@ -1314,23 +1460,15 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
             }
             continue;
         }
-        const double Wf = a.obj_gen[kGen * obj + 0], Yb = a.obj_gen[kGen * obj + 1];
-        const double YY = a.obj_gen[kGen * obj + 2], iW = 1.0 / Wf;
-        double P[16];
-        uint32_t kf[16], flags = 0u;
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            kf[v] = mfma_bin(kbase, rc, kh, v);
+        // Wf, Yb, YY over the FP64 weights w / W of the column factors
+        const double Wf = a.obj_gen[kGen * obj + 5], Yb = a.obj_gen[kGen * obj + 6];
+        const double YY = a.obj_gen[kGen * obj + 7], iW = 1.0 / Wf;
+        mfma_emit<true, OUT>(a, obj, chunk, kbase, [&](int v, bool &ill) {
             double z[6];
-            bool ill;
             sums(v, z);
-            P[v] = gen_power(z[0] * iW, z[1] * iW, 0.5 * (Wf + z[4]) * iW, 0.5 * (Wf - z[4]) * iW, 0.5 * z[5] * iW,
+            return gen_power(z[0] * iW, z[1] * iW, 0.5 * (Wf + z[4]) * iW, 0.5 * (Wf - z[4]) * iW, 0.5 * z[5] * iW,
                              z[2] * iW, z[3] * iW, Yb, YY, ill);
-            if (ill && kf[v] < a.nf) flags |= 1u << v;
-        }
-        recompute_flagged<true, OUT, 16>(
-            a, obj, flags, [&](uint32_t src, int v) { return mfma_bin(kbase, src & 31u, src >> 5, v); }, P);
-        mfma_store<OUT>(a, obj, chunk, kf, P, a.obj_scale[obj], s_pp, s_pi);
+        }, s_pp, s_pi);
     }
 }
 
@@ -1346,13 +1484,13 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
 const TileEntry kSplitTiles[] = {
     // split32 on the matrix cores (standard variant): 64 threads, 32 x 32 bins
     {LSAMD_FORM_SPLIT32, 0, LSAMD_F32, 64, 16, k_sweep_mfma<float>,
-     k_combine_slices<true, false, float, 64, 16, true>, k_colfac<false>},
+     k_combine_slices<true, false, float, 64, 16, true>, k_colfac<false, float>},
     {LSAMD_FORM_SPLIT32, 0, LSAMD_F64, 64, 16, k_sweep_mfma<double>,
-     k_combine_slices<true, false, double, 64, 16, true>, k_colfac<false>},
+     k_combine_slices<true, false, double, 64, 16, true>, k_colfac<false, double>},
     {LSAMD_FORM_SPLIT32, 1, LSAMD_F32, 64, 16, k_sweep_mfma_gen<float>,
-     k_combine_slices<true, true, float, 64, 16, true>, k_colfac<true>},
+     k_combine_slices<true, true, float, 64, 16, true>, k_colfac<true, float>},
     {LSAMD_FORM_SPLIT32, 1, LSAMD_F64, 64, 16, k_sweep_mfma_gen<double>,
-     k_combine_slices<true, true, double, 64, 16, true>, k_colfac<true>},
+     k_combine_slices<true, true, double, 64, 16, true>, k_colfac<true, double>},
     // split form (FP64 rotation and sums), both output types
     LS_SPLIT(0, float, LSAMD_F32, 256, 8), LS_SPLIT(0, double, LSAMD_F64, 256, 8),
     LS_SPLIT(0, float, LSAMD_F32, 256, 4), LS_SPLIT(0, double, LSAMD_F64, 256, 4),
