@@ -792,6 +792,11 @@ def main():
         with open(args.report, "w") as f:
             f.write(full_report + "\n")
     line = json.dumps(compact(out))
+    # the reference's CPU code prints progress text without a final newline
+    # into the C library's stdout buffer: flush it, then start a fresh line
+    import ctypes as ct
+    ct.CDLL(None).fflush(None)
+    sys.stdout.write("\n")
     print(line, flush=True)
 
 

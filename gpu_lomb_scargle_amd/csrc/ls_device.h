@@ -73,8 +73,9 @@ constexpr uint64_t kGroupMinItems = 65536;  // fewer items: one per fetch (e.g. 
 // this many items.
 constexpr uint64_t kSliceTargetItems = 2048;
 // Matrix-core tile: column-factor fragments per 16 observations (standard:
-// y v and w; generalized: q^2 v, q (q y) v and q^2 w; head, tail, negated).
-constexpr int kColfacFragsStd = 12, kColfacFragsGen = 18;
+// y v and w; generalized: (w/W) v, (w/W) y_c v and (w/W) w; real and
+// imaginary head and tail of each).
+constexpr int kColfacFragsStd = 8, kColfacFragsGen = 12;
 // Matrix-core tile: row-step phasors e^{2 pi i 32 a df t} per 16 observations
 // (lane a + 32 h: observations 8 h .. 8 h + 7 as 4 x two complex FP32).
 constexpr int kRowfacFrags = 4;

@@ -46,7 +46,7 @@ struct Workspace {
     size_t queue = 0;       // uint64   persistent-schedule item counter
     size_t part_sums = 0;   // double[K*chunks*slices*NS*M*T] split forms with slices > 1
     size_t obj_ys = 0;      // float[2K]  matrix-core tile: power-of-two y and weight scales
-    size_t colfac = 0;      // matrix-core tile: column-factor fragments, 12 (18 generalized) KiB per 16 observations
+    size_t colfac = 0;      // matrix-core tile: column-factor fragments, 8 (12 generalized) KiB per 16 observations
     size_t rowfac = 0;      // matrix-core tile: row-step phasors, 4 KiB per 16 observations
     uint32_t chunks = 0;
     uint32_t slices = 1;    // observation slices per (object, chunk)

@@ -818,9 +818,10 @@ __device__ __forceinline__ void split64x2(double x0, double x1, half2_t &h, half
 
 // Wavefronts (object, y) with y < gridDim.y write the column-factor fragments
 // of 16-observation blocks y, y + gridDim.y, ... of the object, each factor
-// as head, tail (and negated head, tail where a product needs -f_i):
-//   standard:    y v (6 fragments: r, i, -i), w (6)
-//   generalized: (w/W) v (6), (w/W) y_c v (6), (w/W) w (6)
+// as head and tail of the real and imaginary parts (a product that needs -f_i
+// takes the row factor's negated fragment instead):
+//   standard:    y v (4 fragments), w (4)
+//   generalized: (w/W) v (4), (w/W) y_c v (4), (w/W) w (4)
 // with v = e^{2 pi i b df t} (b < 32 the column) and w = v^2, from the
 // caller's FP64 (or FP32) t, y, dy in FP64 arithmetic: FP64 phase and
 // sin/cos, w/W = 1 / (dy^2 W) the normalised weight, y_c = y - ybar; y (or
@@ -901,7 +902,7 @@ __global__ __launch_bounds__(64) void k_colfac(SweepArgs a) {
         half8_t f[NF];
 #pragma unroll
         for (int v = 0; v < 8; v += 2) {
-            double x[NF / 3][2];  // the factors of observations v, v + 1: real, imaginary per sum
+            double x[NF / 2][2];  // the factors of observations v, v + 1: real, imaginary per sum
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const uint32_t j = q * 16u + 8u * (l >> 5) + (uint32_t)(v + e);
@@ -927,16 +928,14 @@ __global__ __launch_bounds__(64) void k_colfac(SweepArgs a) {
                 }
             }
 #pragma unroll
-            for (int p = 0; p < NF / 6; ++p) {  // each complex factor: r (h, l), i (h, l), -i (h, l)
+            for (int p = 0; p < NF / 4; ++p) {  // each complex factor: r (h, l), i (h, l)
                 half2_t h, t;
                 split64x2(x[2 * p][0], x[2 * p][1], h, t);
-                f[6 * p + 0][v] = h.x; f[6 * p + 0][v + 1] = h.y;
-                f[6 * p + 1][v] = t.x; f[6 * p + 1][v + 1] = t.y;
+                f[4 * p + 0][v] = h.x; f[4 * p + 0][v + 1] = h.y;
+                f[4 * p + 1][v] = t.x; f[4 * p + 1][v + 1] = t.y;
                 split64x2(x[2 * p + 1][0], x[2 * p + 1][1], h, t);
-                f[6 * p + 2][v] = h.x; f[6 * p + 2][v + 1] = h.y;
-                f[6 * p + 3][v] = t.x; f[6 * p + 3][v + 1] = t.y;
-                f[6 * p + 4][v] = -h.x; f[6 * p + 4][v + 1] = -h.y;
-                f[6 * p + 5][v] = -t.x; f[6 * p + 5][v + 1] = -t.y;
+                f[4 * p + 2][v] = h.x; f[4 * p + 2][v + 1] = h.y;
+                f[4 * p + 3][v] = t.x; f[4 * p + 3][v + 1] = t.y;
             }
         }
 #pragma unroll
@@ -956,7 +955,18 @@ __global__ __launch_bounds__(64) void k_colfac(SweepArgs a) {
 // normalisation; the factors are unscaled (|u| <= 1).
 struct RowFrags {
     half8_t ur_h, ur_l, ui_h, ui_l, qr_h, qr_l, qi_h, qi_l;
+    half8_t nui_h, nui_l, nqi_h, nqi_l;  // -u_i, -(u^2)_i: the sign bits flipped
 };
+__device__ __forceinline__ half8_t neg8(half8_t x) {
+    uint4 u;
+    __builtin_memcpy(&u, &x, sizeof(u));
+    u.x ^= 0x80008000u;
+    u.y ^= 0x80008000u;
+    u.z ^= 0x80008000u;
+    u.w ^= 0x80008000u;
+    __builtin_memcpy(&x, &u, sizeof(u));
+    return x;
+}
 __device__ __forceinline__ void row_factors(const float2 *s_u0, uint32_t j0, uint32_t kh,
                                             const uint4 (&rr)[kRowfacFrags], RowFrags &R) {
 #pragma unroll
@@ -989,6 +999,10 @@ __device__ __forceinline__ void row_factors(const float2 *s_u0, uint32_t j0, uin
         R.qi_h[v] = h.x; R.qi_h[v + 1] = h.y;
         R.qi_l[v] = t.x; R.qi_l[v + 1] = t.y;
     }
+    R.nui_h = neg8(R.ui_h);
+    R.nui_l = neg8(R.ui_l);
+    R.nqi_h = neg8(R.qi_h);
+    R.nqi_l = neg8(R.qi_l);
 }
 
 // Loads the row-step fragments of one 16-observation block.
@@ -1189,36 +1203,38 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
                 LS_MFMA(z1r, R.ur_h, col[0]);
                 LS_MFMA(z1r_x, R.ur_h, col[1]);
                 LS_MFMA(z1r_x, R.ur_l, col[0]);
-                LS_MFMA(z1r, R.ui_h, col[4]);
-                LS_MFMA(z1r_x, R.ui_h, col[5]);
-                LS_MFMA(z1r_x, R.ui_l, col[4]);
+                LS_MFMA(z1r, R.nui_h, col[2]);
+                LS_MFMA(z1r_x, R.nui_h, col[3]);
+                LS_MFMA(z1r_x, R.nui_l, col[2]);
                 LS_MFMA(z1i, R.ur_h, col[2]);
                 LS_MFMA(z1i_x, R.ur_h, col[3]);
                 LS_MFMA(z1i_x, R.ur_l, col[2]);
                 LS_MFMA(z1i, R.ui_h, col[0]);
                 LS_MFMA(z1i_x, R.ui_h, col[1]);
                 LS_MFMA(z1i_x, R.ui_l, col[0]);
-                // the same for Z2 with u^2/|u|^2 and w
-                LS_MFMA(z2r, R.qr_h, col[6]);
-                LS_MFMA(z2r, R.qr_h, col[7]);
-                LS_MFMA(z2r, R.qr_l, col[6]);
-                LS_MFMA(z2r, R.qi_h, col[10]);
-                LS_MFMA(z2r, R.qi_h, col[11]);
-                LS_MFMA(z2r, R.qi_l, col[10]);
-                LS_MFMA(z2i, R.qr_h, col[8]);
-                LS_MFMA(z2i, R.qr_h, col[9]);
-                LS_MFMA(z2i, R.qr_l, col[8]);
-                LS_MFMA(z2i, R.qi_h, col[6]);
-                LS_MFMA(z2i, R.qi_h, col[7]);
-                LS_MFMA(z2i, R.qi_l, col[6]);
+                // the same for Z2 with u^2 and w
+                LS_MFMA(z2r, R.qr_h, col[4]);
+                LS_MFMA(z2r, R.qr_h, col[5]);
+                LS_MFMA(z2r, R.qr_l, col[4]);
+                LS_MFMA(z2r, R.nqi_h, col[6]);
+                LS_MFMA(z2r, R.nqi_h, col[7]);
+                LS_MFMA(z2r, R.nqi_l, col[6]);
+                LS_MFMA(z2i, R.qr_h, col[6]);
+                LS_MFMA(z2i, R.qr_h, col[7]);
+                LS_MFMA(z2i, R.qr_l, col[6]);
+                LS_MFMA(z2i, R.qi_h, col[4]);
+                LS_MFMA(z2i, R.qi_h, col[5]);
+                LS_MFMA(z2i, R.qi_l, col[4]);
             }
         }
         if (gend < end) fold();
         }
         // the item's FP64 sums of result entry v
+        // (head x head + cross products in FP32: one rounding relative to the
+        // sum itself)
         auto sums = [&](int v, double &XC, double &XS, double &C2, double &S2) {
-            XC = (double)z1r[v] + (double)z1r_x[v];
-            XS = (double)z1i[v] + (double)z1i_x[v];
+            XC = (double)(z1r[v] + z1r_x[v]);
+            XS = (double)(z1i[v] + z1i_x[v]);
             C2 = (double)z2r[v];
             S2 = (double)z2i[v];
             if (folded) {
@@ -1228,12 +1244,8 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
                 C2 += (double)e[2 * 16 * T];
                 S2 += (double)e[3 * 16 * T];
             }
-            const double uy = kUnscale / (double)ys, uw = kUnscale / (double)ws;  // powers of two
-            XC *= uy;
-            XS *= uy;
-            C2 *= uw;
-            S2 *= uw;
         };
+        const double uy = kUnscale / (double)ys, uw = kUnscale / (double)ws;  // powers of two
         if (a.slices > 1) {
             // the slice's FP64 sums XC, XS, CC, CS of result entry v at
             // [(q M + v) T + l], the matrix-core layout k_combine_slices reads
@@ -1244,18 +1256,25 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mfma(SweepArgs a) {
             for (int v = 0; v < 16; ++v) {
                 double XC, XS, C2, S2;
                 sums(v, XC, XS, C2, S2);
-                dst[(0 * M + v) * T] = XC;
-                dst[(1 * M + v) * T] = XS;
-                dst[(2 * M + v) * T] = 0.5 * (ns + C2);
-                dst[(3 * M + v) * T] = 0.5 * S2;
+                dst[(0 * M + v) * T] = XC * uy;
+                dst[(1 * M + v) * T] = XS * uy;
+                dst[(2 * M + v) * T] = 0.5 * (ns + C2 * uw);
+                dst[(3 * M + v) * T] = 0.5 * (S2 * uw);
             }
             continue;
         }
-        const double nobs = (double)(oend - obeg);
+        // std_power with the scales folded in: CC = (N + uw C2) / 2, CS = uw S2 / 2,
+        // SS = N - CC; the unscaled XC, XS enter the numerator, scaled by uy^2
+        const double nobs = (double)(oend - obeg), hn = 0.5 * nobs, hw = 0.5 * uw;
+        const double dmin = nobs * nobs / (4.0 * kCondMax), hy2 = 0.5 * uy * uy;
         mfma_emit<false, OUT>(a, obj, chunk, kbase, [&](int v, bool &ill) {
             double XC, XS, C2, S2;
             sums(v, XC, XS, C2, S2);
-            return std_power(XC, XS, 0.5 * (nobs + C2), 0.5 * S2, nobs, ill);
+            const double CC = fma(C2, hw, hn), CS = S2 * hw, SS = nobs - CC;
+            const double D = fma(CC, SS, -(CS * CS));
+            ill = !(D >= dmin);  // (CC + SS)^2 / (4 D) > kCondMax, D <= 0 or NaN
+            const double num = fma(SS, XC * XC, fma(CC, XS * XS, -2.0 * CS * (XC * XS)));
+            return fdiv(num * hy2, D);
         }, s_pp, s_pi);
     }
 }
@@ -1373,51 +1392,53 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
                 LS_MFMA(zw[0], R.ur_h, col[0]);
                 LS_MFMA(zw[1], R.ur_h, col[1]);
                 LS_MFMA(zw[1], R.ur_l, col[0]);
-                LS_MFMA(zw[0], R.ui_h, col[4]);
-                LS_MFMA(zw[1], R.ui_h, col[5]);
-                LS_MFMA(zw[1], R.ui_l, col[4]);
+                LS_MFMA(zw[0], R.nui_h, col[2]);
+                LS_MFMA(zw[1], R.nui_h, col[3]);
+                LS_MFMA(zw[1], R.nui_l, col[2]);
                 LS_MFMA(zw[2], R.ur_h, col[2]);
                 LS_MFMA(zw[3], R.ur_h, col[3]);
                 LS_MFMA(zw[3], R.ur_l, col[2]);
                 LS_MFMA(zw[2], R.ui_h, col[0]);
                 LS_MFMA(zw[3], R.ui_h, col[1]);
                 LS_MFMA(zw[3], R.ui_l, col[0]);
-                LS_MFMA(z2[0], R.qr_h, col[12]);
-                LS_MFMA(z2[1], R.qr_h, col[13]);
-                LS_MFMA(z2[1], R.qr_l, col[12]);
-                LS_MFMA(z2[0], R.qi_h, col[16]);
-                LS_MFMA(z2[1], R.qi_h, col[17]);
-                LS_MFMA(z2[1], R.qi_l, col[16]);
-                LS_MFMA(z2[2], R.qr_h, col[14]);
-                LS_MFMA(z2[3], R.qr_h, col[15]);
-                LS_MFMA(z2[3], R.qr_l, col[14]);
-                LS_MFMA(z2[2], R.qi_h, col[12]);
-                LS_MFMA(z2[3], R.qi_h, col[13]);
-                LS_MFMA(z2[3], R.qi_l, col[12]);
-                LS_MFMA(zy[1], R.ur_h, col[7]);
-                LS_MFMA(zy[1], R.ur_l, col[6]);
-                LS_MFMA(zy[1], R.ui_h, col[11]);
-                LS_MFMA(zy[1], R.ui_l, col[10]);
-                LS_MFMA(zy[0], R.ur_h, col[6]);
-                LS_MFMA(zy[0], R.ui_h, col[10]);
-                LS_MFMA(zy[3], R.ur_h, col[9]);
-                LS_MFMA(zy[3], R.ur_l, col[8]);
-                LS_MFMA(zy[3], R.ui_h, col[7]);
-                LS_MFMA(zy[3], R.ui_l, col[6]);
-                LS_MFMA(zy[2], R.ur_h, col[8]);
-                LS_MFMA(zy[2], R.ui_h, col[6]);
+                LS_MFMA(z2[0], R.qr_h, col[8]);
+                LS_MFMA(z2[1], R.qr_h, col[9]);
+                LS_MFMA(z2[1], R.qr_l, col[8]);
+                LS_MFMA(z2[0], R.nqi_h, col[10]);
+                LS_MFMA(z2[1], R.nqi_h, col[11]);
+                LS_MFMA(z2[1], R.nqi_l, col[10]);
+                LS_MFMA(z2[2], R.qr_h, col[10]);
+                LS_MFMA(z2[3], R.qr_h, col[11]);
+                LS_MFMA(z2[3], R.qr_l, col[10]);
+                LS_MFMA(z2[2], R.qi_h, col[8]);
+                LS_MFMA(z2[3], R.qi_h, col[9]);
+                LS_MFMA(z2[3], R.qi_l, col[8]);
+                LS_MFMA(zy[1], R.ur_h, col[5]);
+                LS_MFMA(zy[1], R.ur_l, col[4]);
+                LS_MFMA(zy[1], R.nui_h, col[7]);
+                LS_MFMA(zy[1], R.nui_l, col[6]);
+                LS_MFMA(zy[0], R.ur_h, col[4]);
+                LS_MFMA(zy[0], R.nui_h, col[6]);
+                LS_MFMA(zy[3], R.ur_h, col[7]);
+                LS_MFMA(zy[3], R.ur_l, col[6]);
+                LS_MFMA(zy[3], R.ui_h, col[5]);
+                LS_MFMA(zy[3], R.ui_l, col[4]);
+                LS_MFMA(zy[2], R.ur_h, col[6]);
+                LS_MFMA(zy[2], R.ui_h, col[4]);
             }
         }
         if (gend < end) fold();
         }
         // the item's FP64 sums of result entry v: Re/Im Zw, Zy (y unscaled), Z2
+        // (head x head + cross products in FP32: one rounding relative to the
+        // sum itself)
         auto sums = [&](int v, double (&z)[6]) {
-            z[0] = (double)zw[0][v] + (double)zw[1][v];
-            z[1] = (double)zw[2][v] + (double)zw[3][v];
-            z[2] = (double)zy[0][v] + (double)zy[1][v];
-            z[3] = (double)zy[2][v] + (double)zy[3][v];
-            z[4] = (double)z2[0][v] + (double)z2[1][v];
-            z[5] = (double)z2[2][v] + (double)z2[3][v];
+            z[0] = (double)(zw[0][v] + zw[1][v]);
+            z[1] = (double)(zw[2][v] + zw[3][v]);
+            z[2] = (double)(zy[0][v] + zy[1][v]);
+            z[3] = (double)(zy[2][v] + zy[3][v]);
+            z[4] = (double)(z2[0][v] + z2[1][v]);
+            z[5] = (double)(z2[2][v] + z2[3][v]);
             if (folded) {
                 const float *e = acc_l + v * T;
                 z[0] += (double)e[0 * 16 * T];
@@ -1428,10 +1449,8 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
                 z[2] += d[0];
                 z[3] += d[16 * T];
             }
-            const double uy = kUnscale / (double)ys, uw = kUnscale / (double)ws;  // powers of two
-#pragma unroll
-            for (int q = 0; q < 6; ++q) z[q] *= (q == 2 || q == 3) ? uy : uw;
         };
+        const double uy = kUnscale / (double)ys, uw = kUnscale / (double)ws;  // powers of two
         if (a.slices > 1) {
             // the slice's seven FP64 sums in the matrix-core layout
             // k_combine_slices reads, with the slice's weight sum for CC and SS
@@ -1450,24 +1469,26 @@ __global__ __launch_bounds__(64) void k_sweep_mfma_gen(SweepArgs a) {
             for (int v = 0; v < 16; ++v) {
                 double z[6];
                 sums(v, z);
-                dst[(0 * M + v) * T] = z[0];
-                dst[(1 * M + v) * T] = z[1];
-                dst[(2 * M + v) * T] = z[2];
-                dst[(3 * M + v) * T] = z[3];
-                dst[(4 * M + v) * T] = 0.5 * (wsum + z[4]);
-                dst[(5 * M + v) * T] = 0.5 * (wsum - z[4]);
-                dst[(6 * M + v) * T] = 0.5 * z[5];
+                dst[(0 * M + v) * T] = z[0] * uw;
+                dst[(1 * M + v) * T] = z[1] * uw;
+                dst[(2 * M + v) * T] = z[2] * uy;
+                dst[(3 * M + v) * T] = z[3] * uy;
+                dst[(4 * M + v) * T] = 0.5 * (wsum + z[4] * uw);
+                dst[(5 * M + v) * T] = 0.5 * (wsum - z[4] * uw);
+                dst[(6 * M + v) * T] = 0.5 * (z[5] * uw);
             }
             continue;
         }
         // Wf, Yb, YY over the FP64 weights w / W of the column factors
         const double Wf = a.obj_gen[kGen * obj + 5], Yb = a.obj_gen[kGen * obj + 6];
         const double YY = a.obj_gen[kGen * obj + 7], iW = 1.0 / Wf;
+        const double sw = uw * iW, sy = uy * iW, hsw = 0.5 * sw;  // scales with the weight sum folded in
         mfma_emit<true, OUT>(a, obj, chunk, kbase, [&](int v, bool &ill) {
             double z[6];
             sums(v, z);
-            return gen_power(z[0] * iW, z[1] * iW, 0.5 * (Wf + z[4]) * iW, 0.5 * (Wf - z[4]) * iW, 0.5 * z[5] * iW,
-                             z[2] * iW, z[3] * iW, Yb, YY, ill);
+            const double h2 = z[4] * hsw;  // (Re Z2 / Wf) / 2
+            return gen_power(z[0] * sw, z[1] * sw, 0.5 + h2, 0.5 - h2, z[5] * hsw, z[2] * sy, z[3] * sy, Yb, YY,
+                             ill);
         }, s_pp, s_pi);
     }
 }

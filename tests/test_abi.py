@@ -145,7 +145,7 @@ def test_per_call_tile_overrides_the_default():
 
 def test_matrix_core_tile_workspace():
     """The split32 standard matrix-core tile (64 x 16) reserves its column-
-    factor fragments: 12 fragments x 64 lanes x 16 B per 16-observation block,
+    factor fragments: 8 fragments x 64 lanes x 16 B per 16-observation block,
     in slots floor(beg/16) + k .. + ceil(N/16) per object, which never overlap
     and end below n/16 + K + 1."""
     from ctypes import c_int
@@ -163,9 +163,9 @@ def test_matrix_core_tile_workspace():
         valu = L.lsamd_sweep_workspace_size(byref(a))
     finally:
         L.lsamd_set_tile(a.form, a.variant, 0, 0)
-    colfac = 12 * 64 * 16 * (163652 // 16 + 1000 + 1)
+    colfac = 8 * 64 * 16 * (163652 // 16 + 1000 + 1)
     assert mfma - valu >= colfac   # (its 1024-bin chunks also need more peak partials)
-    # the generalized tile keeps 18 fragments per block (the weights ride on the columns)
+    # the generalized tile keeps 12 fragments per block (the weights ride on the columns)
     a.variant = _native.GENERALIZED
     try:
         gen = L.lsamd_sweep_workspace_size(byref(a))
@@ -173,7 +173,7 @@ def test_matrix_core_tile_workspace():
         gen_valu = L.lsamd_sweep_workspace_size(byref(a))
     finally:
         L.lsamd_set_tile(a.form, a.variant, 0, 0)
-    assert gen - gen_valu >= colfac * 18 // 12
+    assert gen - gen_valu >= colfac * 12 // 8
     assert gen - mfma >= colfac // 2
     # the slot rule on random object sizes (including empty and 1-observation objects)
     rng = np.random.default_rng(5)
