@@ -622,6 +622,9 @@ def main():
     ap.add_argument("--profile-once", action="store_true",
                     help="for rocprofv3 counter runs: one 1 GiB device copy (calibration of the "
                          "HBM byte counters), then one sweep of each form; no JSON")
+    ap.add_argument("--profile-repeat", type=int, default=1,
+                    help="with --profile-once: sweeps of the split32 plans (standard and generalized "
+                         "batches), for steady-state kernel traces")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (a batch would shard "
@@ -656,7 +659,8 @@ def main():
         for form, tile in ((_native.FORM_SPLIT32, None), (_native.FORM_SPLIT32, VALU_TILE),
                            (_native.FORM_SPLIT, None), (_native.FORM_PLAIN, None)):
             plan = make(form, tile)
-            plan.run()
+            for _ in range(args.profile_repeat if form == _native.FORM_SPLIT32 else 1):
+                plan.run()
             torch.cuda.synchronize()
             del plan
         gdata = synth.generalized_batch_1000()
@@ -664,7 +668,8 @@ def main():
                                                    np.float64, dev)
         plan = device.SweepPlan(gt, gy, gdy, goff, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
                                 num_freqs=nf, variant=_native.GENERALIZED, form=_native.FORM_SPLIT32, top_k=1)
-        plan.run()
+        for _ in range(args.profile_repeat):
+            plan.run()
         torch.cuda.synchronize()
         log("profile-once: 1 GiB copy, one standard split32 sweep with each tile (matrix-core, VALU 256x8), "
             "one split and one FP64 plain standard sweep, one generalized split32 sweep")

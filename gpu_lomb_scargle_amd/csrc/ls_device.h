@@ -19,6 +19,7 @@ struct SweepArgs {
     uint64_t item_base;        // first item of this launch
     uint64_t total_items;      // K * chunks
     uint32_t group;            // consecutive items per workgroup fetch (a power of two)
+    uint32_t xcd_run;          // grid schedule: groups kept together on one XCD (0: blockIdx order)
     uint32_t slices;           // split forms: observation slices per (object, chunk), >= 1
     double *part_sums;         // slices > 1: per-slice FP64 sums [K*chunks][slices][NS][M][T]
     unsigned long long *queue; // persistent schedule: next item counter, else null
@@ -347,8 +348,24 @@ __device__ __forceinline__ uint64_t fetch_item(const SweepArgs &a, uint64_t *s_i
     __syncthreads();
     return *s_item;
 }
+// Grid schedule: the 8 XCDs receive workgroups round-robin (workgroup w goes
+// to XCD w % 8), each with its own L2.  Runs of L = xcd_run consecutive
+// groups (about one object's chunks) go to one XCD, and consecutive runs to
+// consecutive XCDs, in alternating direction per round of 8 runs (objects
+// come longest first): workgroup w = 8 i + x takes run 8 r + x' (r = i / L,
+// x' = x, or 7 - x for odd r), group i % L of it; a bijection on the whole
+// rounds of 8 L groups (the rest in blockIdx order).  An object's tables are
+// then fetched into one L2, and the XCDs get equal shares of the work.
+__device__ __forceinline__ uint64_t grid_group(const SweepArgs &a) {
+    const uint64_t w = blockIdx.x, L = a.xcd_run;
+    if (L == 0) return w;
+    const uint64_t round = 8u * L, full = ((uint64_t)gridDim.x / round) * round;
+    if (w >= full) return w;
+    const uint64_t i = w >> 3, r = i / L, x = (r & 1u) ? 7u - (w & 7u) : (w & 7u);
+    return r * round + x * L + i % L;
+}
 __device__ __forceinline__ uint64_t first_item(const SweepArgs &a, uint64_t *s_item) {
-    return a.queue ? fetch_item(a, s_item) : a.item_base + (uint64_t)blockIdx.x * a.group;
+    return a.queue ? fetch_item(a, s_item) : a.item_base + grid_group(a) * a.group;
 }
 __device__ __forceinline__ uint64_t next_item(const SweepArgs &a, uint64_t *s_item, uint64_t item) {
     const uint64_t n = item + 1;

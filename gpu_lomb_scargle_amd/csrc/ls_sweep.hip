@@ -690,6 +690,8 @@ static int sweep_into(const lsamd_sweep_args *a, void *workspace, size_t workspa
     // items to keep every CU busy with a quarter as many workgroups
     sw.group = (split && items >= kGroupMinItems) ? kGroup : 1u;
     const uint64_t groups = (items + sw.group - 1) / sw.group;
+    // about one object's groups per XCD run (grid schedule)
+    sw.xcd_run = (uint32_t)std::max<uint64_t>(1, (uint64_t)w.chunks * w.slices / sw.group);
     const bool persistent = a->schedule == LSAMD_SCHEDULE_PERSISTENT;
     uint32_t pgrid = 0;
     if (persistent) {
