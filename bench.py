@@ -366,6 +366,7 @@ def uneven_accuracy(torch, data, variant, tile, objects, nf, dev):
     r.run()
     torch.cuda.synchronize()
     acc = full_accuracy(torch, p.results(), r.results(), objects)
+    del acc["argmax_identical"]  # counts all three peaks; top1_identical below is per object
     gi, ri = p.results()["index"], r.results()["index"]
     rp = r.results()["power"]
     sep = ((rp[:, :2] - rp[:, 1:]) > 1e-4 * rp[:, :1]).all(dim=1)  # FP64 top-3 powers 1e-4 apart

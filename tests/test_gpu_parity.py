@@ -14,8 +14,8 @@ Tolerances (stated per test):
     1-ulp change of tau flips the float rounding of t - tau at large t;
   * split form vs the FP64 oracle: relative 1e-5 on bins with
     P >= 1e-3 * P_max, |dP| <= 2e-6 * P_max on all bins;
-  * split32 form (FP32 sums): the same on well-conditioned data; relative
-    1e-4 on the aliasing-dominated SDSS data;
+  * split32 form (FP32 sums, bins past the condition bound and objects of
+    at most 16 observations recomputed in FP64): the same;
   * argmax: identical wherever the best two bins differ by more than the
     tolerance; top-k indices exact against the reference rule applied to
     the GPU's own periodogram.
@@ -116,15 +116,15 @@ def test_dropin_gpu_batch_mode1(dtype, error):
 def test_dropin_wrapper_fast_forms(form, error):
     """The reference-compatible wrapper with form= (LSAMD_FORM for the
     drop-in library): FP64 library, SDSS data, against the FP64 oracle at the
-    split forms' tolerances (split32 on these aliased data: 1e-4)."""
+    split forms' tolerance."""
     o, t, y, dy = load_data("SDSS_stripe82_band_z.txt")
     ids, per, pg = ls.lombscargle(o, t, y.copy(), 0.1, 10.0, error, "GPU", dy if error else None,
                                   3000, "double", False, form=form)
     ref, rper = oracle.batch(o, t, y, dy if error else None, TWO_PI * 0.1, TWO_PI * 10.0, 3000, error, F64,
                              nthreads=8)[:2]
     assert "LSAMD_FORM" not in os.environ  # the wrapper restores the environment
-    assert_close(pg, ref, 1e-5 if form == "split" else 1e-4, 1e-3, 2e-6 if form == "split" else 2e-5)
-    assert argmax_agrees(pg, ref, 1e-4) == 0
+    assert_close(pg, ref, 1e-5, 1e-3, 2e-6)
+    assert argmax_agrees(pg, ref, 1e-5) == 0
 
 
 @pytest.mark.parametrize("dtype", ["double", "float"])
@@ -153,9 +153,8 @@ def test_dropin_single_object_fast_forms(form, error):
                                   20000, "double", False, form=form)
     ref = oracle.single(t, y, dy if error else None, TWO_PI * 0.1, TWO_PI * 10.0, 20000, error, F64,
                         nthreads=8)[0]
-    loose = form == "split32" and error   # generalized FP32 sums
-    assert_close(pg[0], ref, 1e-4 if loose else 1e-5, 1e-3, 2e-5 if loose else 2e-6)
-    assert argmax_agrees(pg, ref[None, :], 1e-4) == 0
+    assert_close(pg[0], ref, 1e-5, 1e-3, 2e-6)
+    assert argmax_agrees(pg, ref[None, :], 1e-5) == 0
 
 
 @pytest.mark.parametrize("name,dtype", [("double", F64), ("float", F32)])
@@ -252,7 +251,8 @@ def test_split32_form_vs_fp64_oracle(golden, error, dtype):
 def test_split_forms_on_aliased_sdss_data():
     """SDSS stripe 82 sampling makes the 2x2 normal matrix near-singular at
     the daily aliases (e.g. ~2.003 cycles/day): FP32 sums lose ~1e-4 there,
-    FP64 rotation and sums do not."""
+    FP64 rotation and sums do not.  split32 recomputes the bins past its
+    condition bound in FP64, so both forms meet the same bound."""
     o, t, y, dy = load_data("SDSS_stripe82_band_z.txt")
     ref = oracle.batch(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 5000, True, F64, nthreads=8)
     r = _native.batch_host(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 5000, _native.GENERALIZED,
@@ -260,7 +260,7 @@ def test_split_forms_on_aliased_sdss_data():
     assert_close(r["pgram"], ref[0], 1e-5, 1e-3, 2e-6)
     r = _native.batch_host(o, t, y, dy, TWO_PI * 0.1, TWO_PI * 10.0, 5000, _native.GENERALIZED,
                            F64, _native.FORM_SPLIT32)
-    assert_close(r["pgram"], ref[0], 1e-4, 1e-3, 2e-5)
+    assert_close(r["pgram"], ref[0], 1e-5, 1e-3, 2e-6)
 
 
 def test_split_form_jd_scale_times():
@@ -330,10 +330,7 @@ def test_observation_slices_single_object(form, variant, dtype):
     tol = (1e-12 if dtype == F64 else 2e-7) if form == _native.FORM_SPLIT else 5e-6
     assert_close(p1, pb[0], tol, 1e-3, tol)
     assert np.array_equal(r1["index"].cpu().numpy()[0], rb["index"].cpu().numpy()[0])
-    if form == _native.FORM_SPLIT or not gen:
-        assert_close(p1, ref[0], 1e-5, 1e-3, 2e-6)
-    else:
-        assert_close(p1, ref[0], 1e-4, 1e-3, 2e-5)   # generalized FP32 sums: aliases
+    assert_close(p1, ref[0], 1e-5, 1e-3, 2e-6)
 
 
 @pytest.mark.parametrize("form", [_native.FORM_SPLIT, _native.FORM_SPLIT32])
@@ -432,16 +429,11 @@ def test_ragged_grids_and_staging_boundaries(form, nf):
     elif form == _native.FORM_SPLIT:
         assert_close(pg[ok], ref[0][ok], 1e-5, 1e-3, 2e-6)
     else:
-        # split32 derives SS = N - CC from FP32 phasors whose modulus drifts by
-        # ~1e-7 (hardware sin/cos; the VALU tile's FP32 rotation, the matrix-
-        # core tile's FP16 head/tail factors).  On the 5-observation object
-        # the 2x2 system is nearly singular at some frequencies, and on the
-        # finest grid the VALU tile's error reaches ~1.2e-5 relative and
-        # ~8e-6 P_max (a CPU emulation with correctly rounded sin/cos puts
-        # that object at 6.5e-6 and every other object below 3.5e-6).  Objects
-        # of 64 and more observations meet the 1e-5 relative bound.
+        # split32 computes objects of at most 16 observations (here N = 5, whose
+        # 2x2 system is nearly singular at some frequencies) in FP64, and the
+        # rest with FP32 sums
         big = np.array(counts) >= 64
-        for name, sel, rtol, atol in (("N >= 64", big, 1e-5, 5e-6), ("N = 5", ok & ~big, 3e-5, 1.5e-5)):
+        for name, sel, rtol, atol in (("N >= 64", big, 1e-5, 2e-6), ("N = 5", ok & ~big, 1e-7, 1e-8)):
             print(name, end=": ")
             assert_close(pg[sel], ref[0][sel], rtol, 1e-3, atol)
 
