@@ -892,6 +892,15 @@ size_t lsamd_autotune_workspace_size(const lsamd_sweep_args *a) {
     return need;
 }
 
+// Tiles lsamd_autotune may pick: all compiled shapes, except the VALU tiles of
+// generalized split32 (FP32 rotation and sums), whose worst power error on the
+// 1000-object generalized batch with 2000-day spans is 1.07e-5, above the
+// form's 1e-5 bound (the matrix-core tile: 8.4e-6).  They stay available by
+// explicit choice (tile_threads / tile_fpt, lsamd_set_tile).
+static bool autotune_candidate(const TileEntry &te) {
+    return !(te.form == LSAMD_FORM_SPLIT32 && te.variant == LSAMD_GENERALIZED && te.T != 64);
+}
+
 int lsamd_autotune(const lsamd_sweep_args *a, void *workspace, size_t workspace_bytes,
                    void *stream_v, int *threads_out, int *fpt_out) {
     if (!a) return set_error(LSAMD_ERR_INVALID, "lsamd_autotune: null args");
@@ -918,6 +927,7 @@ int lsamd_autotune(const lsamd_sweep_args *a, void *workspace, size_t workspace_
     for (int i = 0; i < ntab; ++i) {
         const TileEntry &te = tab[i];
         if (te.form != a->form || te.variant != a->variant || te.dtype != a->dtype) continue;
+        if (!autotune_candidate(te)) continue;
         s.tile_threads = te.T;
         s.tile_fpt = te.M;
         Workspace w;

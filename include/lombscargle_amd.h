@@ -181,9 +181,11 @@ void lsamd_graph_destroy(lsamd_graph *graph);
 
 /* Time every compiled tile shape of args->form/variant/dtype on these
  * arguments (reduced to at most 65536 frequency bins, peaks only) and report
- * the fastest; the caller puts it into args->tile_threads / tile_fpt.  No
- * process-wide setting changes.  Uses the caller's workspace; synchronises
- * the stream. */
+ * the fastest; the caller puts it into args->tile_threads / tile_fpt.  For
+ * generalized split32 only the matrix-core tile is a candidate (the VALU
+ * tiles reach 1.07e-5 relative on ill-sampled batches, above the form's
+ * 1e-5 bound; they remain selectable explicitly).  No process-wide setting
+ * changes.  Uses the caller's workspace; synchronises the stream. */
 int lsamd_autotune(const lsamd_sweep_args *args, void *workspace, size_t workspace_bytes,
                    void *stream, int *threads, int *freqs_per_thread);
 /* Workspace bytes lsamd_autotune needs for these arguments. */

@@ -779,6 +779,19 @@ def test_autotune_picks_a_compiled_tile():
     assert_close(plan.results()["pgram"][:20].cpu().numpy(), ref[0], 1e-5, 1e-3, 2e-6)
 
 
+def test_autotune_generalized_split32_keeps_the_matrix_core_tile():
+    """The VALU tiles of generalized split32 are not autotune candidates (their
+    worst error on the 1000-object generalized batch is 1.07e-5, above the
+    form's bound); the matrix-core tile is the only one left."""
+    pytest.importorskip("torch")
+    from gpu_lomb_scargle_amd import device
+    b = synth.lognormal_batch(200, 140, 0.6, 20, 1000, 1800, True, 47)
+    t, y, dy, off = device.numpy_to_device(b["obj_id"], b["t"], b["y"], b["dy"], F64)
+    plan = device.SweepPlan(t, y, dy, off, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
+                            num_freqs=40000, variant=_native.GENERALIZED, form=_native.FORM_SPLIT32, top_k=1)
+    assert plan.autotune() == (64, 16)
+
+
 def test_concurrent_threads_with_different_tiles():
     """Two host threads sweep at the same time on their own streams with
     different per-call tiles (matrix-core and VALU split32) and with kernel
