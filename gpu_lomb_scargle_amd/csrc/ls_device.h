@@ -22,7 +22,7 @@ struct SweepArgs {
     uint32_t xcd_run;          // grid schedule: groups kept together on one XCD (0: blockIdx order)
     uint32_t slices;           // split forms: observation slices per (object, chunk), >= 1
     double *part_sums;         // slices > 1: per-slice FP64 sums [K*chunks][slices][NS][M][T]
-    unsigned long long *queue; // persistent schedule: next item counter, else null
+    unsigned long long *queue; // persistent schedule: kQueues group counters (one per XCD), else null
     // plain (t, y, w); split forms: the caller's y and dy (DTYPE), read by
     // the FP64 recomputation of ill-conditioned bins and small objects
     const void *t, *y, *w, *dy;
@@ -340,11 +340,46 @@ __device__ __forceinline__ void item_coords_sliced(const SweepArgs &a, uint64_t 
 // Work items are numbered in processing order (longest object first),
 // chunk-minor, and taken in aligned groups of a.group consecutive items (the
 // next chunks of the same object, mostly): either one group per workgroup
-// (item_base + blockIdx.x * group), or, with a queue, persistent workgroups
-// that take the next group from a counter until the items run out.
-__device__ __forceinline__ uint64_t fetch_item(const SweepArgs &a, uint64_t *s_item) {
+// (item_base + grid_group * group, below), or, with queues, persistent
+// workgroups that take groups until the items run out.
+//
+// Persistent schedule: kQueues counters, one per XCD.  Groups form runs of
+// L = xcd_run (about one object's chunks); queue q holds runs q, 15 - q,
+// 16 + q, 31 - q, ... (the grid schedule's XCD assignment), so its i-th group
+// is run 8 rho + (rho odd ? 7 - q : q), rho = i / L, offset i % L: every group
+// belongs to exactly one queue, and a queue's groups ascend with i.  A
+// workgroup starts on the queue of its own XCD (blockIdx.x % 8, the dispatch
+// order) and, once that queue is exhausted, takes groups from the next ones in
+// turn, so that an object's tables stay in one XCD's L2 while its work lasts
+// and no XCD idles while another has work.
+constexpr uint32_t kQueues = 8;
+__device__ __forceinline__ uint64_t queue_group(const SweepArgs &a, uint32_t q, uint64_t i) {
+    const uint64_t L = a.xcd_run ? a.xcd_run : 1u;
+    const uint64_t rho = i / L, x = (rho & 1u) ? 7u - q : q;
+    return (8u * rho + x) * L + (i - rho * L);
+}
+__device__ __forceinline__ uint64_t fetch_item(const SweepArgs &a, uint64_t *s_item, bool first) {
+    __shared__ uint32_t s_queue, s_tried;  // this workgroup's current queue, queues found empty
     __syncthreads();  // every thread is done with the previous item and its s_item read
-    if (threadIdx.x == 0) *s_item = atomicAdd(a.queue, (unsigned long long)a.group);
+    if (threadIdx.x == 0) {
+        if (first) {
+            s_queue = blockIdx.x & (kQueues - 1u);
+            s_tried = 0u;
+        }
+        const uint64_t groups = (a.total_items - a.item_base + a.group - 1u) / a.group;
+        uint64_t item = ~0ull;
+        while (s_tried < kQueues) {
+            const uint64_t i = atomicAdd(&a.queue[s_queue], 1ull);
+            const uint64_t g = queue_group(a, s_queue, i);
+            if (g < groups) {
+                item = a.item_base + g * a.group;
+                break;
+            }
+            s_queue = (s_queue + 1u) & (kQueues - 1u);  // exhausted for good: groups ascend with i
+            ++s_tried;
+        }
+        *s_item = item;
+    }
     __syncthreads();
     return *s_item;
 }
@@ -365,12 +400,12 @@ __device__ __forceinline__ uint64_t grid_group(const SweepArgs &a) {
     return r * round + x * L + i % L;
 }
 __device__ __forceinline__ uint64_t first_item(const SweepArgs &a, uint64_t *s_item) {
-    return a.queue ? fetch_item(a, s_item) : a.item_base + grid_group(a) * a.group;
+    return a.queue ? fetch_item(a, s_item, true) : a.item_base + grid_group(a) * a.group;
 }
 __device__ __forceinline__ uint64_t next_item(const SweepArgs &a, uint64_t *s_item, uint64_t item) {
     const uint64_t n = item + 1;
     if (((n - a.item_base) & (uint64_t)(a.group - 1)) != 0) return n;  // same group
-    return a.queue ? fetch_item(a, s_item) : ~0ull;
+    return a.queue ? fetch_item(a, s_item, false) : ~0ull;
 }
 
 template <typename OUT, int T>

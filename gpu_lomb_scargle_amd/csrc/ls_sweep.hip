@@ -287,7 +287,9 @@ __global__ void k_records_split(RecArgs a) {
     }
 }
 
-__global__ void k_zero_counter(unsigned long long *c) { *c = 0ull; }
+__global__ void k_zero_counters(unsigned long long *c, uint32_t n) {
+    if (threadIdx.x < n) c[threadIdx.x] = 0ull;
+}
 
 // ---------------------------------------------------------------------------
 // Merge the chunk partials of each object; period of bin i per the GPU host
@@ -468,7 +470,7 @@ int plan_workspace(const lsamd_sweep_args *a, Workspace *w) {
             w->part_sums = take(8 * base * sl * (size_t)ns * tile.fpt * tile.threads);
         }
     }
-    w->queue = take(8);
+    w->queue = take(8 * kQueues);
     w->bytes = off + 256;
     return LSAMD_OK;
 }
@@ -690,7 +692,7 @@ static int sweep_into(const lsamd_sweep_args *a, void *workspace, size_t workspa
     // items to keep every CU busy with a quarter as many workgroups
     sw.group = (split && items >= kGroupMinItems) ? kGroup : 1u;
     const uint64_t groups = (items + sw.group - 1) / sw.group;
-    // about one object's groups per XCD run (grid schedule)
+    // about one object's groups per XCD run (grid schedule and persistent queues)
     sw.xcd_run = (uint32_t)std::max<uint64_t>(1, (uint64_t)w.chunks * w.slices / sw.group);
     const bool persistent = a->schedule == LSAMD_SCHEDULE_PERSISTENT;
     uint32_t pgrid = 0;
@@ -703,7 +705,7 @@ static int sweep_into(const lsamd_sweep_args *a, void *workspace, size_t workspa
             &per_cu, reinterpret_cast<const void *>(te->fn), T, 0));
         pgrid = (uint32_t)std::max(1, cus * std::max(1, per_cu));
         if ((uint64_t)pgrid > groups) pgrid = (uint32_t)groups;
-        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, sw.queue);
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, sw.queue, kQueues);
         LS_HIP(hipGetLastError());
     }
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
