@@ -28,7 +28,8 @@
 //             observations folded into FP64; the default tiles form the
 //             sums on the FP16 matrix cores as row x column factor products
 //             (head/tail split), folded every 256 observations (standard:
-//             FP32 fold, generalized: FP64 fold).
+//             FP32 fold; generalized: the y sums Zy into FP64, the
+//             weight-only Zw and Z2 into FP32).
 // Both split forms use the tau-free FP64 power
 //     standard:    P = (SS XC^2 - 2 CS XC XS + CC XS^2) / (2 (CC SS - CS^2))
 //     generalized: P = (SS YC^2 - 2 CS YC YS + CC YS^2) / ((CC SS - CS^2) YY)

@@ -82,10 +82,12 @@ enum lsamd_form {
      * and FP32 accumulation, folded every 256 observations (standard: into
      * FP32 sums; generalized: the y sums into FP64, the weight-only sums
      * into FP32); the VALU tiles FP32 rotation / recurrence and FP32 sums
-     * over 32 or 64 observations folded into FP64: the fastest form;
-     * within 1e-5 of FP64 on well-conditioned data (6.4e-6 on a 1000-object
-     * standard batch), up to ~3e-5 on generalized data and ~1e-4 at strong
-     * aliases */
+     * over 32 or 64 observations folded into FP64: the fastest form.
+     * Bins whose 2x2 condition estimate exceeds 16 (aliases) and objects of
+     * at most 16 observations are computed in FP64.  Matrix-core tiles:
+     * within 1e-5 of FP64 on every batch measured (7.9e-6 on a 1000-object
+     * standard batch, 8.4e-6 generalized); the VALU generalized tiles reach
+     * 1.07e-5 on the generalized batch */
     LSAMD_FORM_SPLIT32 = 2
 };
 enum lsamd_schedule {
