@@ -705,12 +705,22 @@ def test_device_plan_matches_host_path():
 # schedules, graph replay, autotuner
 # --------------------------------------------------------------------------
 @pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT, _native.FORM_SPLIT32])
-def test_persistent_schedule_matches_grid_bitwise(form):
+@pytest.mark.parametrize("case", ["300x5003", "70x1024000", "1x100000"])
+def test_persistent_schedule_matches_grid_bitwise(form, case):
+    """The persistent schedule (one group queue per XCD, stealing once a
+    queue is exhausted) visits every work item exactly once: bit-identical to
+    the grid schedule on ungrouped items, on groups of four with a partial
+    last round of queue runs (70,000 items), and on one sliced object whose
+    items all sit in one run (every other queue steals)."""
     torch = pytest.importorskip("torch")
     from gpu_lomb_scargle_amd import device
-    b = synth.lognormal_batch(300, 60, 1.0, 5, 1500, 1800, False, 44)
+    K, nf = (int(x) for x in case.split("x"))
+    if K == 1:
+        b = synth.lognormal_batch(1, 3000, 0.0, 3000, 3000, 1800, False, 45)
+    else:
+        b = synth.lognormal_batch(K, 60, 1.0, 5, 1500, 1800, False, 44)
     t, y, dy, off = device.numpy_to_device(b["obj_id"], b["t"], b["y"], None, F64)
-    kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=5003,
+    kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=nf,
               variant=_native.STANDARD, form=form, top_k=3)
     g = device.SweepPlan(t, y, None, off, schedule=_native.SCHEDULE_GRID, **kw)
     p = device.SweepPlan(t, y, None, off, schedule=_native.SCHEDULE_PERSISTENT, **kw)
