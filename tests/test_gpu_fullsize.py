@@ -15,7 +15,9 @@
   bins is at most twice the oracle's.  Then the split32 form (the
   benchmark's headline form) against that FP64 form over all 1e9 bins:
   relative 1e-5 on bins with P >= 1e-3 P_max, the same best bin for every
-  object.
+  object.  The standard batch also with FP32 inputs and an FP32
+  periodogram (the reference's float build): split32 against the FP64
+  forms on the float-rounded inputs, to the same bar.
 * The uneven batch (log-normal N with median 60, sigma 1, clipped to
   5..5000, which puts ~14% of the objects at N <= 20): its persistent,
   graph-replayed, top-3, no-periodogram split32 path against oracle.top3 of
@@ -76,10 +78,14 @@ def test_config0_single_object_full_size():
         assert int(r["index"][0, 0]) == int(np.argmax(ref))
 
 
-def _batch_full_size(torch, data, variant, oracle_objects):
+def _batch_full_size(torch, data, variant, oracle_objects, dtype=F64):
     from gpu_lomb_scargle_amd import device
     nf = 1000000
     K = int(data["counts"].size)
+    if dtype == np.float32:
+        # the FP64 forms see the float-rounded values the FP32 sweep gets
+        data = dict(data, **{k: data[k].astype(np.float32).astype(F64) for k in ("t", "y", "dy")
+                             if data[k] is not None})
     t, y, dy, off = device.numpy_to_device(data["obj_id"], data["t"], data["y"], data["dy"], F64)
     kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=nf, variant=variant, top_k=1)
     plain = device.SweepPlan(t, y, dy, off, form=_native.FORM_PLAIN, **kw)
@@ -112,20 +118,23 @@ def _batch_full_size(torch, data, variant, oracle_objects):
         assert c["gpu_max"] <= max(2.0 * c["ref_max"], 1e-12)
         assert c["gpu_median"] <= max(2.0 * c["ref_median"], 1e-13)
     del opg, got
+    if dtype == np.float32:
+        t, y, dy = (None if v is None else v.float() for v in (t, y, dy))
     fast = device.SweepPlan(t, y, dy, off, form=_native.FORM_SPLIT32, **kw)
     fast.run()
     torch.cuda.synchronize()
-    res = fast.results()
+    res = {k: (v.double() if v.is_floating_point() else v) for k, v in fast.results().items()}
     err, (ko, kb) = masked_rel(torch, res["pgram"], ref["pgram"], 1e-3)
     same = int((res["index"] == ref["index"]).sum())
-    print(f"split32 vs FP64 over {K} x {nf} bins: max rel {err:.3e} (object {ko}, bin {kb}, "
-          f"N = {int(data['counts'][ko])}), best bins {same}/{K}")
+    print(f"split32 ({np.dtype(dtype).name} inputs) vs FP64 over {K} x {nf} bins: max rel {err:.3e} "
+          f"(object {ko}, bin {kb}, N = {int(data['counts'][ko])}), best bins {same}/{K}")
     return err, same, K
 
 
-def test_config1_standard_batch_full_size():
+@pytest.mark.parametrize("dtype", [F64, np.float32], ids=["fp64_inputs", "fp32_inputs"])
+def test_config1_standard_batch_full_size(dtype):
     torch = pytest.importorskip("torch")
-    err, same, K = _batch_full_size(torch, synth.standard_batch_1000(), _native.STANDARD, 12)
+    err, same, K = _batch_full_size(torch, synth.standard_batch_1000(), _native.STANDARD, 12, dtype)
     assert err <= 1e-5 and same == K
 
 
