@@ -15,9 +15,10 @@
   bins is at most twice the oracle's.  Then the split32 form (the
   benchmark's headline form) against that FP64 form over all 1e9 bins:
   relative 1e-5 on bins with P >= 1e-3 P_max, the same best bin for every
-  object.  The standard batch also with FP32 inputs and an FP32
-  periodogram (the reference's float build): split32 against the FP64
-  forms on the float-rounded inputs, to the same bar.
+  object.  The standard batch, and the generalized batch with dmag per
+  observation, also with FP32 inputs and an FP32 periodogram (the
+  reference's float builds): split32 against the FP64 forms on the
+  float-rounded inputs, to the same bar.
 * The uneven batch (log-normal N with median 60, sigma 1, clipped to
   5..5000, which puts ~14% of the objects at N <= 20): its persistent,
   graph-replayed, top-3, no-periodogram split32 path against oracle.top3 of
@@ -138,11 +139,12 @@ def test_config1_standard_batch_full_size(dtype):
     assert err <= 1e-5 and same == K
 
 
-@pytest.mark.parametrize("per_obs", [False, True], ids=["dmag_per_object", "dmag_per_observation"])
-def test_config2_generalized_batch_full_size(per_obs):
+@pytest.mark.parametrize("per_obs,dtype", [(False, F64), (True, F64), (True, np.float32)],
+                         ids=["dmag_per_object", "dmag_per_observation", "dmag_per_observation_fp32_inputs"])
+def test_config2_generalized_batch_full_size(per_obs, dtype):
     torch = pytest.importorskip("torch")
     data = synth.generalized_batch_1000(dmag_per_obs=per_obs)
-    err, same, K = _batch_full_size(torch, data, _native.GENERALIZED, 6)
+    err, same, K = _batch_full_size(torch, data, _native.GENERALIZED, 6, dtype)
     assert err <= 1e-5 and same == K
 
 
