@@ -33,6 +33,14 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Lane src's v, in every lane (src the same in all lanes).
+__device__ __forceinline__ double lane_value(double v, int src) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // sin and cos of 2 pi r for |r| <= 1/2 revolution: the octant q = rint(4 r)
 // is taken off exactly, and x = 2 pi (r - q/4), |x| <= pi/4, goes through the
 // Taylor series of sin to x^17 and of cos to x^16 (the first omitted terms
@@ -181,7 +189,8 @@ __device__ __forceinline__ void recompute_flagged(const SweepArgs &a, uint32_t o
 // matrix-core tile's layout, bin kbase + rc + 32 (8 (v / 4) + 4 kh + v % 4),
 // for an object of at most kSmallObs observations: per observation one FP64
 // phasor at the lane's first bin and FP64 rotations by 32 and 256 bins
-// (sums as exact_power's).
+// (sums as exact_power's).  All 64 lanes call it (the rotations are read
+// across lanes).
 template <bool GEN, typename IN>
 __device__ void small_object_powers(const SweepArgs &a, uint32_t obj, uint32_t kbase, uint32_t rc,
                                     uint32_t kh, double (&P)[16]) {
@@ -201,6 +210,13 @@ __device__ void small_object_powers(const SweepArgs &a, uint32_t obj, uint32_t k
     for (int q = 0; q < NS; ++q)
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc[q][v] = 0.0;
+    // The rotations by 32 and 256 bins depend on the observation alone: lane
+    // i < 16 evaluates observation b + i's by 32 bins and lane 16 + i its by
+    // 256, once, and the loop reads them from those lanes.
+    static_assert(kSmallObs <= 16, "one lane per observation and rotation");
+    const uint32_t lane = threadIdx.x & 63u, li = lane & 15u;
+    double rotc = 1.0, rots = 0.0;
+    if (lane < 32u && b + li < e) phasor64(lane < 16u ? f32 : f256, a.rec_t[b + li], rotc, rots);
     double W = 0.0, WY = 0.0, WYY = 0.0;
     for (uint64_t j = b; j < e; ++j) {
         const double t = a.rec_t[j];
@@ -215,10 +231,11 @@ __device__ void small_object_powers(const SweepArgs &a, uint32_t obj, uint32_t k
         } else {
             yv = (double)y[j];
         }
-        double zc, zs, r1c, r1s, r8c, r8s;
+        double zc, zs;
         phasor64(f0, t, zc, zs);
-        phasor64(f32, t, r1c, r1s);
-        phasor64(f256, t, r8c, r8s);
+        const int k = (int)(j - b);  // < kSmallObs
+        const double r1c = lane_value(rotc, k), r1s = lane_value(rots, k);
+        const double r8c = lane_value(rotc, 16 + k), r8s = lane_value(rots, 16 + k);
         const double wy = w * yv;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
