@@ -124,7 +124,10 @@ typedef struct lsamd_sweep_args {
     uint64_t num_obs;     /* total observations n = offsets[K] */
     double min_freq;      /* angular (rad/day) */
     double max_freq;      /* angular, excluded */
-    const uint64_t *offsets; /* device, K+1: object k holds rows [offsets[k], offsets[k+1]) */
+    const uint64_t *offsets; /* device, K+1: object k holds rows [offsets[k], offsets[k+1]);
+                              * an empty object gets NaN powers (0/0) and peaks at bin 0
+                              * with power 0, and leaves the other objects' results as
+                              * they are without it */
     const uint32_t *order;   /* device, K: processing order (longest first is best), or NULL */
     const void *t;           /* device, n */
     const void *y;           /* device, n (not modified) */

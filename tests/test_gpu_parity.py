@@ -674,6 +674,36 @@ def test_identical_times_zero_sum_gives_zero_power(form):
 # --------------------------------------------------------------------------
 # device-tensor API (torch supplies memory and stream)
 # --------------------------------------------------------------------------
+@pytest.mark.parametrize("variant", [_native.STANDARD, _native.GENERALIZED], ids=["standard", "generalized"])
+@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT, _native.FORM_SPLIT32])
+def test_empty_objects_in_a_device_batch(form, variant):
+    """Caller-built offsets may hold empty objects, which the reference's row
+    runs cannot express.  Such an object's periodogram is 0/0 (NaN), as the
+    reference's arithmetic gives for a one-observation object, and its peaks
+    stay at bin 0 with power 0 (NaN never enters a peak list).  The other
+    objects come out bit-identical to the same batch without the empty ones."""
+    torch = pytest.importorskip("torch")
+    from gpu_lomb_scargle_amd import device
+    rng = np.random.default_rng(29)
+    counts = [40, 0, 25, 0, 7, 300, 0]
+    kept = [k for k, c in enumerate(counts) if c]
+    n = sum(counts)
+    t = np.sort(rng.uniform(0, 1800, n))
+    y = np.sin(2.1 * t) + rng.normal(0, 0.1, n)
+    dy = rng.uniform(0.01, 0.2, n)
+    tt, yy, dd = (torch.tensor(v, device="cuda") for v in (t, y, dy))
+    dd = dd if variant == _native.GENERALIZED else None
+    kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=3001, variant=variant, form=form,
+              top_k=3)
+    r = device.sweep(tt, yy, dd, device.offsets_from_counts(counts, "cuda"), **kw)
+    s = device.sweep(tt, yy, dd, device.offsets_from_counts([counts[k] for k in kept], "cuda"), **kw)
+    torch.cuda.synchronize()
+    empty = [k for k, c in enumerate(counts) if not c]
+    pg = r["pgram"].cpu().numpy()
+    assert np.isnan(pg[empty]).all()
+    assert (r["index"][empty] == 0).all() and (r["power"][empty] == 0).all()
+    for key in ("pgram", "index", "power", "period"):
+        assert np.array_equal(r[key][kept].cpu().numpy(), s[key].cpu().numpy()), key
 def test_device_plan_matches_host_path():
     torch = pytest.importorskip("torch")
     from gpu_lomb_scargle_amd import device
