@@ -704,6 +704,41 @@ def test_empty_objects_in_a_device_batch(form, variant):
     assert (r["index"][empty] == 0).all() and (r["power"][empty] == 0).all()
     for key in ("pgram", "index", "power", "period"):
         assert np.array_equal(r[key][kept].cpu().numpy(), s[key].cpu().numpy()), key
+
+
+@pytest.mark.parametrize("variant", [_native.STANDARD, _native.GENERALIZED], ids=["standard", "generalized"])
+@pytest.mark.parametrize("form", [_native.FORM_PLAIN, _native.FORM_SPLIT, _native.FORM_SPLIT32])
+def test_nan_magnitude_stays_in_its_object(form, variant):
+    """A NaN magnitude makes its object's powers NaN, as in the reference's
+    arithmetic (its normalisation and sums propagate it), and the reference's
+    strict-maximum scan then keeps bin 0 with power 0.  The other objects are
+    bit-identical to the same batch without the NaN."""
+    torch = pytest.importorskip("torch")
+    from gpu_lomb_scargle_amd import device
+    rng = np.random.default_rng(31)
+    counts = [40, 25, 7, 300]
+    n = sum(counts)
+    t = np.sort(rng.uniform(0, 1800, n))
+    y = np.sin(2.1 * t) + rng.normal(0, 0.1, n)
+    dy = rng.uniform(0.01, 0.2, n)
+    off = device.offsets_from_counts(counts, "cuda")
+    kw = dict(min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX, num_freqs=3001, variant=variant, form=form,
+              top_k=3)
+    dd = torch.tensor(dy, device="cuda") if variant == _native.GENERALIZED else None
+    tt = torch.tensor(t, device="cuda")
+    s = device.sweep(tt, torch.tensor(y, device="cuda"), dd, off, **kw)
+    y[40 + 3] = np.nan  # object 1
+    r = device.sweep(tt, torch.tensor(y, device="cuda"), dd, off, **kw)
+    torch.cuda.synchronize()
+    gen = variant == _native.GENERALIZED
+    ref = oracle.batch(np.repeat(np.arange(4, dtype=np.uint32), counts), t, y.copy(), dy if gen else None,
+                       synth.OMEGA_MIN, synth.OMEGA_MAX, 3001, gen, F64)
+    assert np.isnan(ref[0][1]).all() and ref[3][1] == 0  # the reference's CPU path, restated
+    assert np.isnan(r["pgram"][1].cpu().numpy()).all()
+    assert (r["index"][1] == 0).all() and (r["power"][1] == 0).all()
+    for key in ("pgram", "index", "power", "period"):
+        for k in (0, 2, 3):
+            assert np.array_equal(r[key][k].cpu().numpy(), s[key][k].cpu().numpy()), (key, k)
 def test_device_plan_matches_host_path():
     torch = pytest.importorskip("torch")
     from gpu_lomb_scargle_amd import device
