@@ -571,6 +571,42 @@ def test_split32_matrix_core_tile_y_range(scale):
     assert_close(pg[keep], ref[keep], 1e-5, 1e-3, 2e-6)
 
 
+@pytest.mark.parametrize("variant", [_native.STANDARD, _native.GENERALIZED], ids=["standard", "generalized"])
+def test_long_objects_in_a_batch(variant):
+    """Light curves of 65536 and 20000 observations (Kepler long-cadence
+    scale) beside short ones: a batch's objects are not sliced, so the
+    matrix-core tile folds 256 of its 256-observation partials in FP32 for
+    the longest.  The plain FP64 form against the oracle on the longest
+    object; the split forms against the plain form on every object (1e-5 and
+    2e-6 relative on bins with P >= 1e-3 P_max, the best bins where
+    separated)."""
+    gen = variant == _native.GENERALIZED
+    rng = np.random.default_rng(41)
+    counts = [65536, 20000, 5000, 140]
+    ts, ys, ds = [], [], []
+    for n in counts:
+        t = np.sort(rng.integers(0, 1801, n) + rng.uniform(0, 0.3, n))
+        y = np.sin(2 * np.pi * t / np.exp(rng.uniform(np.log(0.05), np.log(10))) + rng.uniform(0, 6.28))
+        y = y + rng.normal(0, 0.3, n)
+        ts.append(t)
+        ys.append(17.0 + 0.3 * y if gen else y)
+        ds.append(np.exp(rng.uniform(np.log(0.01), np.log(0.2), n)))
+    o = np.repeat(np.arange(len(counts), dtype=np.uint32), counts)
+    t, y, dy = np.concatenate(ts), np.concatenate(ys), np.concatenate(ds)
+    dy = dy if gen else None
+    nf = 10007
+    plain = _native.batch_host(o, t, y, dy, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, variant, F64, _native.FORM_PLAIN)
+    ref0 = oracle.batch(o[:counts[0]], t[:counts[0]], y[:counts[0]].copy(), None if dy is None else dy[:counts[0]],
+                        synth.OMEGA_MIN, synth.OMEGA_MAX, nf, gen, F64, nthreads=16)[0]
+    print("plain FP64 vs oracle, N = 65536", end=": ")
+    assert_close(plain["pgram"][:1], ref0, 1e-9 if gen else 1e-12, 1e-3, 1e-10 if gen else 1e-12)
+    for form, tol, atol in ((_native.FORM_SPLIT, 2e-6, 1e-6), (_native.FORM_SPLIT32, 1e-5, 2e-6)):
+        r = _native.batch_host(o, t, y, dy, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, variant, F64, form)
+        print(f"form {form} vs plain FP64", end=": ")
+        assert_close(r["pgram"], plain["pgram"], tol, 1e-3, atol)
+        assert argmax_agrees(r["pgram"], plain["pgram"], 1e-5) == 0
+
+
 def test_many_tiny_objects():
     b = synth.lognormal_batch(3000, 8, 0.5, 5, 40, 1800, False, 5)
     ref = oracle.batch(b["obj_id"], b["t"], b["y"], None, synth.OMEGA_MIN, synth.OMEGA_MAX, 300,
