@@ -19,6 +19,9 @@
   observation, also with FP32 inputs and an FP32 periodogram (the
   reference's float builds): split32 against the FP64 forms on the
   float-rounded inputs, to the same bar.
+* The plain FP32 form (the reference's float build) at 1e6 bins on the
+  first 8 objects of each batch: bit-identical to the float oracle with
+  correctly rounded sin/cos/atan2.
 * The uneven batch (log-normal N with median 60, sigma 1, clipped to
   5..5000, which puts ~14% of the objects at N <= 20): its persistent,
   graph-replayed, top-3, no-periodogram split32 path against oracle.top3 of
@@ -137,6 +140,30 @@ def test_config1_standard_batch_full_size(dtype):
     torch = pytest.importorskip("torch")
     err, same, K = _batch_full_size(torch, synth.standard_batch_1000(), _native.STANDARD, 12, dtype)
     assert err <= 1e-5 and same == K
+
+
+@pytest.mark.parametrize("variant", [_native.STANDARD, _native.GENERALIZED], ids=["standard", "generalized"])
+def test_plain_fp32_bit_identical_at_full_grid(variant):
+    """The plain FP32 form (the reference's float build: float phases,
+    frequencies and sums) at the full 1e6 bins, where float phases reach
+    ~3e5 rad and any change in the order of the float operations would show:
+    on the first 8 objects of the 1000-object batch (the generalized one
+    with dmag per observation), every power and best bin bit-identical to
+    the float oracle run with correctly rounded sin/cos/atan2 (the GPU
+    evaluates those of float arguments correctly rounded)."""
+    gen = variant == _native.GENERALIZED
+    d = synth.generalized_batch_1000(dmag_per_obs=True) if gen else synth.standard_batch_1000()
+    rows = int(d["counts"][:8].sum())
+    o, t, y = d["obj_id"][:rows], d["t"][:rows], d["y"][:rows]
+    dy = d["dy"][:rows] if gen else None
+    nf = 1000000
+    r = _native.batch_host(o, t, y, dy, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, variant, np.float32, _native.FORM_PLAIN)
+    with oracle.float_libm_cr():
+        ref = oracle.batch(o, t, y.copy(), dy, synth.OMEGA_MIN, synth.OMEGA_MAX, nf, gen, np.float32, nthreads=16)
+    same = float((r["pgram"] == ref[0]).mean())
+    print(f"plain FP32, 8 objects x {nf} bins: bit-identical fraction {same:.6f}")
+    assert r["pgram"].dtype == np.float32 and same == 1.0
+    assert np.array_equal(r["index"][:, 0], ref[3])
 
 
 @pytest.mark.parametrize("per_obs,dtype", [(False, F64), (True, F64), (True, np.float32)],
