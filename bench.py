@@ -626,6 +626,9 @@ def main():
     ap.add_argument("--profile-repeat", type=int, default=1,
                     help="with --profile-once: sweeps of the split32 plans (standard and generalized "
                          "batches), for steady-state kernel traces")
+    ap.add_argument("--profile-uneven", type=int, default=0, metavar="K",
+                    help="for rocprofv3 runs: K eager persistent split32 sweeps (matrix-core tile, top-3, "
+                         "no periodogram) of the 100000-object uneven batch per variant; no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (a batch would shard "
@@ -638,6 +641,21 @@ def main():
 
     dev = torch.device("cuda:0")
     L = _native.core()
+    if args.profile_uneven:
+        for variant in (_native.STANDARD, _native.GENERALIZED):
+            u = synth.uneven_batch_100000(with_error=variant == _native.GENERALIZED)
+            ut, uy, udy, uoff = device.numpy_to_device(u["obj_id"], u["t"], u["y"], u["dy"], np.float64, dev)
+            plan = device.SweepPlan(ut, uy, udy, uoff, min_freq=synth.OMEGA_MIN, max_freq=synth.OMEGA_MAX,
+                                    num_freqs=200000, variant=variant, form=_native.FORM_SPLIT32, top_k=3,
+                                    pgram=False, schedule=_native.SCHEDULE_PERSISTENT, tile=MFMA_TILE)
+            for _ in range(args.profile_uneven):
+                plan.run()
+            torch.cuda.synchronize()
+            log(f"profile-uneven: {args.profile_uneven} persistent split32 sweeps, variant {variant}, "
+                f"{int(u['counts'].sum())} rows x 200000 freqs")
+            del plan, ut, uy, udy, uoff
+            torch.cuda.empty_cache()
+        return
     cpu = host_cpu()
     cpu_threads = args.cpu_threads or cpu["threads"]
     data = synth.standard_batch_1000()
